@@ -1,0 +1,214 @@
+// Median detrend (gfilter) and the on-device offset (get_offset) for gfx950.
+//
+// Reference semantics: gtrap/gfilter.py:13-85.  For each output row i the
+// reference forms the window x~[i-r .. i+r] with zero padding outside
+// 0 <= i < n (FILVAL 0.0, gfilter.py:12,25-36), selects its exact median by
+// value bisection (gfilter.py:40-74) and writes x[i] - median (gfilter.py:76).
+// Any exact selection returns the same window element, so this file uses a
+// different selector: every lane owns one (curve, time segment) and keeps the
+// window sorted in registers while it slides along time.  One slide step
+// deletes the outgoing sample and inserts the incoming one with 3 branch-free
+// VALU ops per slot (compare, select, v_med3_f32); the median is the centre
+// slot.  Windows narrower than the register array are padded with -inf/+inf
+// sentinels so the median always sits at compile-time index RMAX.
+//
+// Lanes are numbered g = segment*cols + curve, so for the interleaved (n, nq)
+// image consecutive lanes read consecutive curves: one wave reads 256
+// contiguous bytes per row (coalesced), unlike the reference's (1,1000)
+// blocks that read one curve per block with stride nq.
+#include "common.h"
+#include <math.h>
+
+namespace gtrap {
+
+// Delete `ov` (known to be present) from the sorted array s and insert `nv`.
+template <int W>
+__device__ __forceinline__ void slide_sorted(float (&s)[W], float ov, float nv) {
+  float d[W];
+#pragma unroll
+  for (int k = 0; k < W - 1; ++k) d[k] = (s[k] < ov) ? s[k] : s[k + 1];
+  d[W - 1] = (s[W - 1] < ov) ? s[W - 1] : INFINITY;
+  s[0] = fminf(nv, d[0]);
+#pragma unroll
+  for (int k = 1; k < W; ++k) s[k] = med3(d[k - 1], nv, d[k]);
+}
+
+template <int RMAX>
+__global__ __launch_bounds__(256) void medfilt_win_kernel(
+    const float *__restrict__ x, float *__restrict__ y, int n, int nq, int r, int mode,
+    int rows_out, int cols_out, int seg, int nseg) {
+  constexpr int W = 2 * RMAX + 1;
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (long)nseg * cols_out) return;
+  const int q = (int)(g % cols_out);
+  const int sidx = (int)(g / cols_out);
+  const int i0 = sidx * seg;
+  const int i1 = min(i0 + seg, rows_out);
+  if (i0 >= i1) return;
+
+  float s[W];
+#pragma unroll
+  for (int k = 0; k < W; ++k)
+    s[k] = (k < RMAX - r) ? -INFINITY : ((k > RMAX + r) ? INFINITY : 0.0f);
+
+  const float *xq = x + q;
+  auto ld = [&](int i) -> float { return (i >= 0 && i < n) ? xq[(size_t)i * nq] : 0.0f; };
+
+  // The 2r+1 middle slots start as placeholder zeros; replace them one by one
+  // with the window of row i0.
+  for (int j = -r; j <= r; ++j) slide_sorted<W>(s, 0.0f, ld(i0 + j));
+
+  float *yq = y + q;
+  for (int i = i0; i < i1; ++i) {
+    const float med = s[RMAX];
+    const float xv = ld(i);
+    yq[(size_t)i * nq] = mode ? xv / med : xv - med;
+    if (i + 1 < i1) slide_sorted<W>(s, ld(i - r), ld(i + r + 1));
+  }
+}
+
+// Order-preserving map float -> uint32 (total order, -0 < +0).
+__device__ __forceinline__ uint32_t fkey(float v) {
+  uint32_t b = __float_as_uint(v);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float fkey_inv(uint32_t k) {
+  uint32_t b = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+  return __uint_as_float(b);
+}
+
+// Wide windows (r > 64): exact (r+1)-th smallest by bisection over the 32-bit
+// key space, 32 counting passes over the window per output.
+__global__ __launch_bounds__(256) void medfilt_wide_kernel(
+    const float *__restrict__ x, float *__restrict__ y, int n, int nq, int r, int mode,
+    int rows_out, int cols_out) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (long)rows_out * cols_out) return;
+  const int q = (int)(g % cols_out);
+  const int i = (int)(g / cols_out);
+  const int lo = i - r, hi = i + r;
+  uint32_t key = 0;  // smallest key K with count(key <= K) >= r+1
+  for (int bit = 31; bit >= 0; --bit) {
+    const uint32_t trial = key | ((1u << bit) - 1u);  // all lower bits set
+    int cnt = 0;
+    for (int j = lo; j <= hi; ++j) {
+      const float v = (j >= 0 && j < n) ? x[(size_t)j * nq + q] : 0.0f;
+      cnt += (fkey(v) <= trial) ? 1 : 0;
+    }
+    if (cnt < r + 1) key |= (1u << bit);
+  }
+  const float med = fkey_inv(key);
+  const float xv = x[(size_t)i * nq + q];
+  y[(size_t)i * nq + q] = mode ? xv / med : xv - med;
+}
+
+// Per-curve masked mean over n (gtrap/gfilter.py:94-101): one block per curve,
+// float64 accumulation, fixed reduction tree (deterministic).
+__global__ __launch_bounds__(256) void offset_kernel(const float *__restrict__ img,
+                                                     const float *__restrict__ tu, int n,
+                                                     int nq, double *off64, float *off32,
+                                                     uint32_t *ntrue) {
+  const int q = blockIdx.x;
+  double acc = 0.0;
+  uint32_t cnt = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const size_t e = (size_t)i * nq + q;
+    if (tu[e] >= 0.0f) {
+      acc += (double)img[e];
+      cnt += 1;
+    }
+  }
+  __shared__ double sacc[256];
+  __shared__ uint32_t scnt[256];
+  sacc[threadIdx.x] = acc;
+  scnt[threadIdx.x] = cnt;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sacc[threadIdx.x] += sacc[threadIdx.x + s];
+      scnt[threadIdx.x] += scnt[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double m = sacc[0] / (double)n;
+    if (off64) off64[q] = m;
+    if (off32) off32[q] = (float)m;
+    if (ntrue) ntrue[q] = scnt[0];
+  }
+}
+
+template <int RMAX>
+static void launch_win(const float *in, float *out, int n, int nq, int r, int mode,
+                       int rows_out, int cols_out, int seg, int nseg, hipStream_t st) {
+  const long lanes = (long)nseg * cols_out;
+  const int block = 256;
+  const long grid = (lanes + block - 1) / block;
+  hipLaunchKernelGGL((medfilt_win_kernel<RMAX>), dim3((unsigned)grid), dim3(block), 0, st, in,
+                     out, n, nq, r, mode, rows_out, cols_out, seg, nseg);
+}
+
+// Register-window buckets: the smallest RMAX >= r is used.
+static const int kBuckets[] = {4, 8, 16, 24, 32, 48, 64};
+
+int medfilt_launch(const float *in, float *out, uint32_t n, uint32_t nq, uint32_t r, int mode,
+                   uint32_t rows_out, uint32_t cols_out, uint32_t seg_len, hipStream_t st) {
+  GTRAP_REQUIRE(in && out, "gtrap_medfilt: null pointer");
+  GTRAP_REQUIRE(mode == GTRAP_MEDFILT_SUB || mode == GTRAP_MEDFILT_DIV,
+                "gtrap_medfilt: mode must be 0 (sub) or 1 (div)");
+  GTRAP_REQUIRE(rows_out <= n && cols_out <= nq, "gtrap_medfilt: rows_out/cols_out exceed n/nq");
+  GTRAP_REQUIRE(n < (1u << 30) && nq < (1u << 30) && r < (1u << 28),
+                "gtrap_medfilt: size out of range");
+  GTRAP_REQUIRE(in != out, "gtrap_medfilt: in-place filtering is not supported");
+  if (rows_out == 0 || cols_out == 0) return GTRAP_OK;
+  const int w = 2 * (int)r + 1;
+  if (r > 64) {
+    const long total = (long)rows_out * cols_out;
+    hipLaunchKernelGGL(medfilt_wide_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       st, in, out, (int)n, (int)nq, (int)r, mode, (int)rows_out,
+                       (int)cols_out);
+    return check_launch("medfilt_wide_kernel");
+  }
+  int seg = (int)seg_len;
+  if (seg <= 0) {
+    // Enough lanes to fill 256 CUs x 4 SIMDs x 2 waves, but at least 4 windows
+    // per segment so the initial fill is amortised.
+    const long target = 131072;
+    long per = (long)cols_out >= target ? 1 : (target + cols_out - 1) / cols_out;
+    seg = (int)(((long)rows_out + per - 1) / per);
+    if (seg < 4 * w) seg = 4 * w;
+  }
+  if (seg > (int)rows_out) seg = (int)rows_out;
+  const int nseg = ((int)rows_out + seg - 1) / seg;
+  int rmax = 64;
+  for (int b : kBuckets)
+    if ((int)r <= b) { rmax = b; break; }
+  switch (rmax) {
+    case 4: launch_win<4>(in, out, n, nq, r, mode, rows_out, cols_out, seg, nseg, st); break;
+    case 8: launch_win<8>(in, out, n, nq, r, mode, rows_out, cols_out, seg, nseg, st); break;
+    case 16: launch_win<16>(in, out, n, nq, r, mode, rows_out, cols_out, seg, nseg, st); break;
+    case 24: launch_win<24>(in, out, n, nq, r, mode, rows_out, cols_out, seg, nseg, st); break;
+    case 32: launch_win<32>(in, out, n, nq, r, mode, rows_out, cols_out, seg, nseg, st); break;
+    case 48: launch_win<48>(in, out, n, nq, r, mode, rows_out, cols_out, seg, nseg, st); break;
+    default: launch_win<64>(in, out, n, nq, r, mode, rows_out, cols_out, seg, nseg, st); break;
+  }
+  return check_launch("medfilt_win_kernel");
+}
+
+}  // namespace gtrap
+
+extern "C" int gtrap_medfilt(const float *in, float *out, uint32_t n, uint32_t nq, uint32_t r,
+                             int mode, uint32_t rows_out, uint32_t cols_out, uint32_t seg_len,
+                             void *stream) {
+  return gtrap::medfilt_launch(in, out, n, nq, r, mode, rows_out, cols_out, seg_len,
+                               gtrap::as_stream(stream));
+}
+
+extern "C" int gtrap_offset(const float *img, const float *tu, uint32_t n, uint32_t nq,
+                            double *offset64, float *offset32, uint32_t *ntrue, void *stream) {
+  GTRAP_REQUIRE(img && tu, "gtrap_offset: null pointer");
+  GTRAP_REQUIRE(n > 0 && nq > 0, "gtrap_offset: empty image");
+  hipLaunchKernelGGL(gtrap::offset_kernel, dim3(nq), dim3(256), 0, gtrap::as_stream(stream), img,
+                     tu, (int)n, (int)nq, offset64, offset32, ntrue);
+  return gtrap::check_launch("offset_kernel");
+}
