@@ -104,7 +104,75 @@ def bench_gfilter(steps: int, warmup: int):
     return res
 
 
-WORKLOADS = {"gfilter": bench_gfilter}
+# ----------------------------------------------------------------------------
+# geebls workload: 1024 curves x 16384 samples, 2048 frequencies, nb 1024
+# ----------------------------------------------------------------------------
+def bench_geebls(steps: int, warmup: int):
+    import torch
+    from gtrap_amd import geebls, getstat, synth
+    log("geebls: generating 1024 curves x 16384 samples")
+    d = synth.config2()
+    g = synth.config2_grid()
+    lc, tu, ntrue = d["lc"], d["tu"], d["ntrue"]
+    n, nq = lc.shape
+    nf = g["nf"]
+    off = (np.where(tu >= 0, lc, 0).astype(np.float64).sum(0) / n).astype(np.float32)
+    dl, dt, do, dn = (torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (lc, tu, off, ntrue))
+    blsp = torch.empty(nq * nf, dtype=torch.float32, device="cuda")
+    p1 = torch.empty(nq * nf, dtype=torch.int32, device="cuda")
+    p2 = torch.empty(nq * nf, dtype=torch.int32, device="cuda")
+    evs = []
+
+    def step(record):
+        if record:
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record()
+        geebls.geebls_device(blsp, p1, p2, dl, dt, do, dn, n, nq, nf, g["nb"], g["kma"], g["kmi"],
+                             g["kkmi"], g["fmin"], g["df"], nthread=g["nthread"])
+        if record:
+            b.record()
+            evs.append((a, b))
+        getstat.blsstats_device(blsp, p1, p2, nf, nq, g["df"], g["fmin"])
+
+    for _ in range(warmup):
+        step(False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step(True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+    trials = nf * nq * g["nb"] * (g["kma"] - g["kmi"])
+    lds_bytes = nf * nq * (8.0 * n + 8.0 * (g["nb"] + g["kma"]))
+    log(f"geebls kernel {kms:.3f} ms; trial evals/s {trials / kms * 1e3:.3e}")
+    cores, model = cpu_info()
+    from oracle import bls as ob
+    t1 = time.perf_counter()
+    nfs = 0
+    while time.perf_counter() - t1 < 10.0 and nfs < nf:
+        ob.geebls(lc, tu, off, ntrue, nf, g["nb"], g["kma"], g["kmi"], g["kkmi"], g["fmin"],
+                  g["df"], nthread=g["nthread"], qs=(0, 1), fs=(nfs, nfs + 16))
+        nfs += 16
+    cpu_s_per_curve = (time.perf_counter() - t1) / nfs * nf
+    ach = lds_bytes / (kms * 1e-3) / 1e12
+    lds_peak = 256 * 128 * 2.4e9 / 1e12
+    return dict(
+        metric="light curves/s and trial evals/s (geebls + getstat)",
+        value=nq * steps / wall, unit="light curves/s", ms_per_step=wall / steps * 1e3,
+        dtype="f32", trial_evals_per_s=trials / (kms * 1e-3),
+        config=dict(workload="batched geebls 1024 curves x 16384 samples, 2048 frequencies, "
+                             "nb 1024, 12 widths", n=n, nq=nq, nf=nf, nb=g["nb"]),
+        roofline=dict(bound="lds", achieved=ach, peak=lds_peak, unit="TB/s",
+                      frac=ach / lds_peak, traffic=None, kernel="bls_kernel", kernel_ms=kms),
+        cpu_baseline=dict(value=1.0 / cpu_s_per_curve, unit="light curves/s", cores=1,
+                          kind="port", sample=f"{nfs} frequencies of 1 curve with the C oracle, "
+                                              f"scaled to {nf} ({model})"),
+    )
+
+
+WORKLOADS = {"gfilter": bench_gfilter, "geebls": bench_geebls}
 
 
 def main(argv=None):

@@ -20,6 +20,12 @@ SIGNATURES: dict[str, tuple] = {
     "gtrap_version": (C.c_char_p, []),
     "gtrap_medfilt": (_int, [_vp, _vp, _u32, _u32, _u32, _int, _u32, _u32, _u32, _vp]),
     "gtrap_offset": (_int, [_vp, _vp, _u32, _u32, _vp, _vp, _vp, _vp]),
+    "gtrap_geebls": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _u32, _u32, _u32, _u32,
+                            _u32, _u32, _f32, _f32, _u32, _vp]),
+    "gtrap_geebls_simple": (_int, [_vp, _vp, _vp, _u32, _u32, _u32, _u32, _u32, _u32, _u32,
+                                   _f32, _f32, _u32, _vp]),
+    "gtrap_blsstats": (_int, [_vp, _vp, _vp, _u32, _u32, _u32, _f64, _f64, _vp, _vp, _vp, _vp,
+                              _vp, _vp]),
 }
 
 _lib = None

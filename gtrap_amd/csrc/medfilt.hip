@@ -33,6 +33,8 @@ __device__ __forceinline__ void slide_sorted(float (&s)[W], float ov, float nv) 
   for (int k = 1; k < W; ++k) s[k] = med3(d[k - 1], nv, d[k]);
 }
 
+constexpr int kUnroll = 8;
+
 template <int RMAX>
 __global__ __launch_bounds__(256) void medfilt_win_kernel(
     const float *__restrict__ x, float *__restrict__ y, int n, int nq, int r, int mode,
@@ -55,15 +57,51 @@ __global__ __launch_bounds__(256) void medfilt_win_kernel(
   auto ld = [&](int i) -> float { return (i >= 0 && i < n) ? xq[(size_t)i * nq] : 0.0f; };
 
   // The 2r+1 middle slots start as placeholder zeros; replace them one by one
-  // with the window of row i0.
-  for (int j = -r; j <= r; ++j) slide_sorted<W>(s, 0.0f, ld(i0 + j));
+  // with the window of row i0 (loads issued kUnroll at a time).
+  for (int j = -r; j <= r; j += kUnroll) {
+    float v[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) v[u] = (j + u <= r) ? ld(i0 + j + u) : 0.0f;
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u)
+      if (j + u <= r) slide_sorted<W>(s, 0.0f, v[u]);
+  }
 
+  // Main sweep, software-pipelined: the loads of the next kUnroll rows are in
+  // flight while the current rows are filtered (the sweep is otherwise bound
+  // by one dependent global-load latency per row).
   float *yq = y + q;
-  for (int i = i0; i < i1; ++i) {
-    const float med = s[RMAX];
-    const float xv = ld(i);
-    yq[(size_t)i * nq] = mode ? xv / med : xv - med;
-    if (i + 1 < i1) slide_sorted<W>(s, ld(i - r), ld(i + r + 1));
+  float cin[kUnroll], cout_[kUnroll], cc[kUnroll];
+#pragma unroll
+  for (int u = 0; u < kUnroll; ++u) {
+    cc[u] = ld(i0 + u);
+    cin[u] = ld(i0 + u + r + 1);
+    cout_[u] = ld(i0 + u - r);
+  }
+  for (int i = i0; i < i1; i += kUnroll) {
+    float nin[kUnroll], nout[kUnroll], nc[kUnroll];
+    const int in_ = i + kUnroll;
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const bool more = in_ < i1;
+      nc[u] = more ? ld(in_ + u) : 0.0f;
+      nin[u] = more ? ld(in_ + u + r + 1) : 0.0f;
+      nout[u] = more ? ld(in_ + u - r) : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      if (i + u < i1) {
+        const float med = s[RMAX];
+        yq[(size_t)(i + u) * nq] = mode ? cc[u] / med : cc[u] - med;
+        slide_sorted<W>(s, cout_[u], cin[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      cc[u] = nc[u];
+      cin[u] = nin[u];
+      cout_[u] = nout[u];
+    }
   }
 }
 
@@ -171,9 +209,9 @@ int medfilt_launch(const float *in, float *out, uint32_t n, uint32_t nq, uint32_
   }
   int seg = (int)seg_len;
   if (seg <= 0) {
-    // Enough lanes to fill 256 CUs x 4 SIMDs x 2 waves, but at least 4 windows
-    // per segment so the initial fill is amortised.
-    const long target = 131072;
+    // Enough lanes for 256 CUs x 4 SIMDs x 4 waves, but at least 4 windows per
+    // segment so the initial fill is amortised.
+    const long target = 262144;
     long per = (long)cols_out >= target ? 1 : (target + cols_out - 1) / cols_out;
     seg = (int)(((long)rows_out + per - 1) / per);
     if (seg < 4 * w) seg = 4 * w;

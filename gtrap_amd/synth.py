@@ -58,57 +58,59 @@ def config1(seed: int = 1001):
 
 def config2(nq: int = 1024, n: int = 16384, seed: int = 1002):
     """Batched geebls workload: non-common time (1% cadence jitter per curve),
-    5% masked in 3 gaps (time set to INVAL, flux zig-zag filled with the
-    curve's min/max as gtrap/read_keplerlc.py:39-46), AR(1) red noise
-    (phi 0.9, sigma 5e-4) plus white noise 1e-3, boxes in half the curves."""
+    5% of samples masked in 3 gaps (time set to INVAL, flux zig-zag filled
+    with the curve's min/max as gtrap/read_keplerlc.py:39-46), AR(1) red noise
+    (phi 0.9, innovation sigma 5e-4) plus white noise 1e-3, box transits
+    (depth U(5e-4, 5e-3), P U(1, 30) d, duration fraction U(11, 22)/1024 of P)
+    in a random half of the curves.  Returns lc, tu (n, nq), ntrue, truth."""
+    from scipy.signal import lfilter
     rng = np.random.default_rng(seed)
     lc = np.empty((n, nq), np.float32)
     tu = np.empty((n, nq), np.float32)
     ntrue = np.empty(nq, np.uint32)
     truth = []
     nmask = int(round(0.05 * n))
+    even = (np.arange(n) % 2) == 0
     for q in range(nq):
         dt = CADENCE * (1.0 + rng.uniform(-0.01, 0.01))
         t = np.arange(n) * dt
-        e = rng.normal(0, 5e-4, n)
-        red = np.empty(n)
-        acc = 0.0
-        for i in range(n):  # AR(1)
-            acc = 0.9 * acc + e[i]
-            red[i] = acc
+        red = lfilter([1.0], [1.0, -0.9], rng.normal(0, 5e-4, n))
         f = 1.0 + red + rng.normal(0, 1e-3, n)
-        P = None
+        item = None
         if rng.random() < 0.5:
             P = rng.uniform(1.0, 30.0)
-            depth = rng.uniform(5e-4, 5e-3)
-            qdur = rng.uniform(11, 22) / 1024
-            f -= box_dip(t, rng.uniform(0, P), P, depth, qdur * P)
-        truth.append(P)
-        # three non-overlapping gaps totalling nmask samples
+            item = dict(P=P, depth=rng.uniform(5e-4, 5e-3), q=rng.uniform(11, 22) / 1024,
+                        epoch=rng.uniform(0, P))
+            f -= box_dip(t, item["epoch"], P, item["depth"], item["q"] * P)
+        truth.append(item)
+        # three non-overlapping gaps whose lengths sum to nmask
         cuts = np.sort(rng.choice(np.arange(1, nmask), 2, replace=False))
         lens = np.diff(np.concatenate([[0], cuts, [nmask]]))
-        starts = np.sort(rng.choice(np.arange(0, n - nmask - 3), 3, replace=False))
-        starts = starts + np.arange(3) * 0  # positions in the gap-free index space
+        slots = np.sort(rng.choice(n - nmask, 3, replace=False))
         mask = np.zeros(n, bool)
-        pos = 0
-        free = np.arange(n)
-        offs = 0
-        for s0, L in zip(starts, lens):
-            a = s0 + offs
-            mask[a:a + L] = True
-            offs += L
-        tt = t.copy()
-        tt[mask] = INVAL
-        ff = f.copy()
-        lo, hi = ff[~mask].min(), ff[~mask].max()
-        m2 = mask.copy()
-        ev = np.zeros(n, bool); ev[::2] = True
-        ff[m2 & ev] = lo
-        ff[m2 & ~ev] = hi
-        lc[:, q] = ff
+        shift = 0
+        for s0, L in zip(slots, lens):
+            mask[s0 + shift:s0 + shift + L] = True
+            shift += L
+        tt = np.where(mask, INVAL, t)
+        lo, hi = f[~mask].min(), f[~mask].max()
+        f = np.where(mask & even, lo, np.where(mask & ~even, hi, f))
+        lc[:, q] = f
         tu[:, q] = tt
         ntrue[q] = int((~mask).sum())
     return dict(lc=lc, tu=tu, ntrue=ntrue, truth=truth)
+
+
+def config2_grid(n: int = 16384):
+    """Trial grid of the batched geebls workload: fmin = 1/30, nf = 2048
+    frequencies up to 1 d^-1, nb = 1024 bins, widths kmi < k <= kma with
+    kmi = 10, kma = 22 (12 widths), kkmi = int(n*kmi/nb), block 512."""
+    nf, nb = 2048, 1024
+    fmin = 1.0 / 30.0
+    df = (1.0 - fmin) / (nf - 1)
+    kmi, kma = 10, 22
+    return dict(nf=nf, nb=nb, kma=kma, kmi=kmi, kkmi=int(n * kmi / nb), fmin=fmin, df=df,
+                nthread=512)
 
 
 def config3(nq: int = 4096, n: int = 65536, seed: int = 1003, device=None):

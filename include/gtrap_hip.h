@@ -61,6 +61,52 @@ int gtrap_medfilt(const float *in, float *out, uint32_t n, uint32_t nq, uint32_t
 int gtrap_offset(const float *img, const float *tu, uint32_t n, uint32_t nq,
                  double *offset64, float *offset32, uint32_t *ntrue, void *stream);
 
+/* ---- box least squares (geebls / geebls_simple) ------------------------- */
+
+/*
+ * Replaces the geebls kernel of include/geebls.h:1-167 (module factory
+ * gtrap/geebls.py:10-22, MINBIN 5, INVT 0.0).  One trial frequency
+ * f0 = fmin + df*jf per (jf < nf) and curve q < nq:
+ *   fold samples i < floor(n/nthread)*nthread with tu >= 0 into nb phase bins,
+ *   bin sums of (lc - offset[q]) and counts, wrap kma bins, scan start bin x width,
+ *   power = s^2 / (kk (ntrue[q] - kk)) for width > max(kmi,1) and kk > max(kkmi,5),
+ *   blsp = sqrt(max power), phase1/phase2 = start/end bin of the best box.
+ * Outputs are curve-major: index jf + nf*q.  nthread is the reference's
+ * blockDim.x (it decides the dropped sample tail, include/geebls.h:57).
+ * phase1/phase2 may be NULL.  When no box qualifies, blsp = 0 and the phases are
+ * 0xFFFFFFFF.  Ties resolve to the lowest (start, end) pair.
+ */
+int gtrap_geebls(float *blsp, uint32_t *phase1, uint32_t *phase2,
+                 const float *lc, const float *tu, const float *offset,
+                 const uint32_t *ntrue, uint32_t n, uint32_t nq, uint32_t nf,
+                 uint32_t nb, uint32_t kma, uint32_t kmi, uint32_t kkmi,
+                 float fmin, float df, uint32_t nthread, void *stream);
+
+/*
+ * Replaces the geebls kernel of gtrap/geebls_simple.py:17-175: common 1-D time
+ * array tu[n], no mask, no offset, N = n, blsp only.
+ */
+int gtrap_geebls_simple(float *blsp, const float *lc, const float *tu,
+                        uint32_t n, uint32_t nq, uint32_t nf, uint32_t nb,
+                        uint32_t kma, uint32_t kmi, uint32_t kkmi,
+                        float fmin, float df, uint32_t nthread, void *stream);
+
+/* ---- BLS statistics (getstat) -------------------------------------------- */
+
+/*
+ * Replaces gtrap/getstat.py:14-123 (getmax kernel + patch_stat + combine_stat):
+ * per curve q, over the first npatch*1024 frequencies (npatch = floor(nf/1024)),
+ *   maxv = max blsp, idx = lowest index of the max,
+ *   Pest = 1/(idx*df + fmin), ave = sum/nf, var = sumsq/nf - ave^2,
+ *   sde = (maxv - ave)/sqrt(var), phasebest = phase1/2[idx].
+ * Sums in float64.  phase1/phase2/phasebest1/phasebest2 may be NULL.
+ * patch: the reference's nthread (1024 by default, gtrap/getstat.py:64).
+ */
+int gtrap_blsstats(const float *blsp, const uint32_t *phase1, const uint32_t *phase2,
+                   uint32_t nf, uint32_t nq, uint32_t patch, double df, double fmin,
+                   double *pest, double *sde, uint32_t *phasebest1,
+                   uint32_t *phasebest2, float *maxval, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,159 @@
+"""GPU parity of geebls / geebls_simple / getstat against the oracle.
+
+The GPU kernel and the C restatement perform the same float32 operations in
+the same order on the same exact fixed-point bin sums, so SR and phases must
+match bit for bit.
+"""
+import numpy as np
+import pytest
+
+from oracle import bls as ob
+from oracle import getstat as og
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _small(n=2048, nq=5, seed=0):
+    from gtrap_amd import synth
+    d = synth.config2(nq=nq, n=n, seed=seed)
+    lc, tu, ntrue = d["lc"], d["tu"], d["ntrue"]
+    off = np.where(tu >= 0, lc, 0).astype(np.float64).sum(0) / n
+    return lc, tu, off.astype(np.float32), ntrue
+
+
+def _run_geebls(lc, tu, off, ntrue, nf, nb, kma, kmi, kkmi, fmin, df, nthread):
+    import torch
+    from gtrap_amd import geebls
+    n, nq = lc.shape
+    blsp = torch.full((nq * nf,), -1.0, dtype=torch.float32, device="cuda")
+    p1 = torch.zeros(nq * nf, dtype=torch.int32, device="cuda")
+    p2 = torch.zeros(nq * nf, dtype=torch.int32, device="cuda")
+    geebls.geebls_device(blsp, p1, p2, _dev(lc), _dev(tu), _dev(off), _dev(ntrue), n, nq, nf, nb,
+                         kma, kmi, kkmi, fmin, df, nthread=nthread)
+    torch.cuda.synchronize()
+    return (blsp.cpu().numpy().reshape(nq, nf), p1.cpu().numpy().view(np.uint32).reshape(nq, nf),
+            p2.cpu().numpy().view(np.uint32).reshape(nq, nf))
+
+
+@pytest.mark.parametrize("nb,kma,kmi,nthread", [(1024, 22, 10, 512), (256, 30, 2, 128),
+                                                (64, 8, 0, 64), (2048, 103, 10, 1024)])
+def test_geebls_bit_exact(gpu, nb, kma, kmi, nthread):
+    lc, tu, off, ntrue = _small(n=3000, nq=4, seed=nb)
+    nf, fmin, df = 37, 1.0 / 30, 0.011
+    kkmi = int(3000 * kmi / nb)
+    got = _run_geebls(lc, tu, off, ntrue, nf, nb, kma, kmi, kkmi, fmin, df, nthread)
+    want = ob.geebls(lc, tu, off, ntrue, nf, nb, kma, kmi, kkmi, fmin, df, nthread=nthread)
+    for g, w, name in zip(got, want, ("blsp", "phase1", "phase2")):
+        np.testing.assert_array_equal(g, w, err_msg=name)
+
+
+def test_geebls_simple_bit_exact(gpu):
+    import torch
+    from gtrap_amd import geebls_simple
+    rng = np.random.default_rng(1)
+    n, nq, nf, nb, kma, kmi, kkmi = 4096, 3, 50, 512, 40, 3, 20
+    t = (np.arange(n) * 0.0204).astype(np.float32)
+    lc = (1e-3 * rng.standard_normal((n, nq))).astype(np.float32)
+    blsp = torch.zeros(nq * nf, dtype=torch.float32, device="cuda")
+    geebls_simple.geebls_simple_device(blsp, _dev(lc), _dev(t), n, nq, nf, nb, kma, kmi, kkmi,
+                                       0.05, 0.013, nthread=256)
+    torch.cuda.synchronize()
+    want, _, _ = ob.geebls(lc, t, None, None, nf, nb, kma, kmi, kkmi, 0.05, 0.013, nthread=256,
+                           simple=True)
+    np.testing.assert_array_equal(blsp.cpu().numpy().reshape(nq, nf), want)
+
+
+def test_geebls_dropin_module_and_getstat(gpu):
+    """Reference call shape (include/geebls.h:1 via gbls_module) then get_blsstat."""
+    from gtrap_amd import driver as cuda
+    from gtrap_amd import geebls, getstat
+    lc, tu, off, ntrue = _small(n=4096, nq=3, seed=7)
+    n, nq = lc.shape
+    nf, nb, qmi, qma = 2048, 1024, 0.01, 0.1
+    kma, kmi, kkmi, shared = geebls.bls_params(n, nb, qmi, qma)
+    fmin, df = 1.0 / 30, 3e-4
+    dev = {k: cuda.mem_alloc(v.nbytes) for k, v in
+           dict(lc=lc, tu=tu, off=off, ntrue=ntrue).items()}
+    for k, v in dict(lc=lc, tu=tu, off=off, ntrue=ntrue).items():
+        cuda.memcpy_htod(dev[k], v)
+    blsp = cuda.mem_alloc(4 * nf * nq)
+    p1 = cuda.mem_alloc(4 * nf * nq)
+    p2 = cuda.mem_alloc(4 * nf * nq)
+    k = geebls.gbls_module().get_function("geebls")
+    k(blsp, p1, p2, dev["lc"], dev["tu"], dev["off"], dev["ntrue"], np.uint32(n), np.uint32(nf),
+      np.uint32(nb), np.uint32(kma), np.uint32(kmi), np.uint32(kkmi), np.float32(fmin),
+      np.float32(df), block=(512, 1, 1), grid=(nf, nq), shared=shared)
+    hb = np.zeros(nf * nq, np.float32)
+    h1 = np.zeros(nf * nq, np.uint32)
+    h2 = np.zeros(nf * nq, np.uint32)
+    cuda.memcpy_dtoh(hb, blsp)
+    cuda.memcpy_dtoh(h1, p1)
+    cuda.memcpy_dtoh(h2, p2)
+    fs = (0, 96)
+    want, w1, w2 = ob.geebls(lc, tu, off, ntrue, nf, nb, kma, kmi, kkmi, fmin, df, nthread=512,
+                             fs=fs)
+    np.testing.assert_array_equal(hb.reshape(nq, nf)[:, :96], want[:, :96])
+    np.testing.assert_array_equal(h1.reshape(nq, nf)[:, :96], w1[:, :96])
+    np.testing.assert_array_equal(h2.reshape(nq, nf)[:, :96], w2[:, :96])
+    pest, sde, pb1, pb2 = getstat.get_blsstat(blsp, p1, p2, nf, nq, df, fmin)
+    opest, osde, op1, op2, _ = og.blsstat(hb, h1, h2, nf, nq, df, fmin)
+    np.testing.assert_array_equal(pest, opest)
+    np.testing.assert_allclose(sde, osde, rtol=1e-12)
+    np.testing.assert_array_equal(pb1, op1)
+    np.testing.assert_array_equal(pb2, op2)
+
+
+def test_config2_full_batch_subset(gpu):
+    """1024 curves x 16384 samples, 2048 frequencies, nb 1024, kma 22: the whole
+    batch on the GPU, a subset of (curve, frequency) cells against the oracle,
+    and box recovery on the injected curves."""
+    import torch
+    from gtrap_amd import geebls, getstat, synth
+    d = synth.config2()
+    g = synth.config2_grid()
+    lc, tu, ntrue = d["lc"], d["tu"], d["ntrue"]
+    n, nq = lc.shape
+    off = (np.where(tu >= 0, lc, 0).astype(np.float64).sum(0) / n).astype(np.float32)
+    nf = g["nf"]
+    blsp = torch.empty(nq * nf, dtype=torch.float32, device="cuda")
+    p1 = torch.empty(nq * nf, dtype=torch.int32, device="cuda")
+    p2 = torch.empty(nq * nf, dtype=torch.int32, device="cuda")
+    geebls.geebls_device(blsp, p1, p2, _dev(lc), _dev(tu), _dev(off), _dev(ntrue), n, nq, nf,
+                         g["nb"], g["kma"], g["kmi"], g["kkmi"], g["fmin"], g["df"],
+                         nthread=g["nthread"])
+    pest, sde, _, _, _ = getstat.blsstats_device(blsp, p1, p2, nf, nq, g["df"], g["fmin"])
+    torch.cuda.synchronize()
+    hb = blsp.cpu().numpy().reshape(nq, nf)
+    rng = np.random.default_rng(5)
+    for q in rng.choice(nq, 6, replace=False):
+        fs = sorted(rng.choice(nf - 8, 1))[0]
+        want, _, _ = ob.geebls(lc, tu, off, ntrue, nf, g["nb"], g["kma"], g["kmi"], g["kkmi"],
+                               g["fmin"], g["df"], nthread=g["nthread"], qs=(q, q + 1),
+                               fs=(fs, fs + 8))
+        np.testing.assert_array_equal(hb[q, fs:fs + 8], want[q, fs:fs + 8])
+    opest, osde, _, _, _ = og.blsstat(hb, None, None, nf, nq, g["df"], g["fmin"])
+    np.testing.assert_array_equal(pest.cpu().numpy(), opest)
+    np.testing.assert_allclose(sde.cpu().numpy(), osde, rtol=1e-10)
+    # deep boxes are recovered at the true period or a harmonic
+    got = pest.cpu().numpy()
+    hits = tot = 0
+    for q, tr in enumerate(d["truth"]):
+        if tr is None or tr["depth"] < 3e-3:
+            continue
+        tot += 1
+        ratio = got[q] / tr["P"]
+        hits += any(abs(ratio * h - 1) < 0.01 for h in (1, 2, 3, 0.5, 1 / 3))
+    assert tot > 50 and hits / tot > 0.8, (hits, tot)
+
+
+def test_invalid_arguments_raise(gpu):
+    lc, tu, off, ntrue = _small(n=512, nq=1)
+    with pytest.raises(ValueError):
+        _run_geebls(lc, tu, off, ntrue, 4, 1000, 8, 1, 5, 0.1, 0.01, 500)  # nb not pow2
+    with pytest.raises(ValueError):
+        _run_geebls(lc, tu, off, ntrue, 4, 1024, 600, 1, 5, 0.1, 0.01, 512)  # kma > nthread
