@@ -26,6 +26,10 @@ SIGNATURES: dict[str, tuple] = {
                                    _f32, _f32, _u32, _vp]),
     "gtrap_blsstats": (_int, [_vp, _vp, _vp, _u32, _u32, _u32, _f64, _f64, _vp, _vp, _vp, _vp,
                               _vp, _vp]),
+    "gtrap_gtls": (_int, [_vp] * 7 + [_i32] * 5 + [_f32] * 5 + [_int, _vp]),
+    "gtrap_tlsfold_ws_bytes": (_i64, [_i32, _i32]),
+    "gtrap_tlsfold": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _i32,
+                             _i64, _vp, _i32, _i32, _f64, _vp, _i64, _i32, _vp]),
 }
 
 _lib = None

@@ -1,0 +1,523 @@
+// Periodic trapezoid least squares (tls_fold) for gfx950.
+//
+// Statistic: gtrap/tls.py:173-208 (get_chi2Hmax, divx=True, sigma=1) with
+// S/N = -H/sqrt(res/(n-3)) (tls.py:229), evaluated on time folded at each
+// trial period.  Definition (identical in the oracle):
+//   u = fmod(t, P) (float64, exact), epoch e < ceil(P/dtep) at t0 = e*dtep,
+//   d = u - t0, or (u - P) - t0 if d >= P/2, or (u + P) - t0 if d < -P/2;
+//   L = m * (0.45*W/nl); regions A..E of tls.py:175-179 on d; mean over
+//   [-W/2, W/2]; best trial per period = first maximum in (W, L, epoch)
+//   order with NaN skipped (tlsfit, tls.py:217-231).
+//
+// MI355X design.  One workgroup owns one (curve, period) work item at a time
+// (persistent grid, per-block global workspace):
+//  1. fold + bucket sort: u of every valid sample is bucketed into ~n/2
+//     uniform phase buckets (global atomics), scanned, scattered, and each
+//     bucket insertion-sorted, giving the folded curve sorted by phase;
+//  2. epoch tiles: the union of the windows [t0 - Wmax/2, t0 + Wmax/2] of a
+//     tile of epochs is a contiguous range of the (cyclically extended)
+//     sorted curve; it is staged in LDS with float64 prefix sums of
+//     (x', x'^2, q, q^2, x'q), q = phase relative to the tile, x' = x - xref;
+//  3. one thread per (epoch, W): region boundaries are found by binary search
+//     with the exact float64 offset test, region sums are prefix differences
+//     shifted to the epoch's coordinates, and each L gives H, res and S/N in
+//     O(1).  Windows larger than the LDS tile are swept in chunks with the
+//     boundary sums accumulated per (W, boundary).
+#include "common.h"
+#include <math.h>
+
+namespace gtrap {
+
+constexpr int kTfThreads = 256;
+constexpr int kTfChunk = 512;       // samples staged in LDS per tile/chunk
+constexpr int kTfMaxNl = 12;
+constexpr int kTfMaxNw = 16;
+constexpr int kTfMaxBnd = 3 + 4 * kTfMaxNl;  // boundaries per W
+
+// Loads of words updated by global atomics go to L2 (agent scope), never to a
+// possibly stale L1 line.
+__device__ __forceinline__ int ld_agent(const int *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+struct TfBest {
+  double sn, h;
+  int w, l, e;
+};
+
+// better: larger S/N, ties to the earliest (w, l, e); w < 0 means "none yet"
+__device__ __forceinline__ bool tf_better(double sn, int w, int l, int e, const TfBest &b) {
+  if (sn > b.sn) return true;
+  if (sn == b.sn && b.w >= 0)
+    return (w < b.w) || (w == b.w && (l < b.l || (l == b.l && e < b.e)));
+  return false;
+}
+
+// Region boundary offsets of width W and step m (tls.py:175-179):
+// kind 0: first index with d >= c ; kind 1: first index with d > c.
+__device__ __forceinline__ void tf_bounds(double W, int m, int nl, double &tAB, double &tBC,
+                                          double &tCD, double &tDE) {
+  const double dl = 0.45 * W / (double)nl;
+  const double L = (double)m * dl;
+  const double qw = 0.25 * W, hl = 0.5 * L;
+  tAB = -qw - hl;
+  tBC = -qw + hl;
+  tCD = qw - hl;
+  tDE = qw + hl;
+}
+
+// Image of extended index j of the sorted folded curve (nv samples):
+// j < 0 -> u[j+nv] - P, j >= nv -> u[j-nv] + P.
+__device__ __forceinline__ double tf_img(const double *su, int nv, double P, int j) {
+  if (j < 0) return su[j + nv] - P;
+  if (j >= nv) return su[j - nv] + P;
+  return su[j];
+}
+__device__ __forceinline__ int tf_src(int nv, int j) {
+  return j < 0 ? j + nv : (j >= nv ? j - nv : j);
+}
+
+// first extended index j in [lo, hi) with img(j) - t0 >= c (kind 0) or > c (kind 1)
+__device__ int tf_search_global(const double *su, int nv, double P, int lo, int hi, double t0,
+                                double c, int kind) {
+  while (lo < hi) {
+    const int mid = lo + ((hi - lo) >> 1);
+    const double d = tf_img(su, nv, P, mid) - t0;
+    const bool pred = kind ? (d > c) : (d >= c);
+    if (pred) hi = mid; else lo = mid + 1;
+  }
+  return lo;
+}
+
+// same over an LDS array of images
+__device__ __forceinline__ int tf_search(const double *lu, int lo, int hi, double t0, double c,
+                                         int kind) {
+  while (lo < hi) {
+    const int mid = lo + ((hi - lo) >> 1);
+    const double d = lu[mid] - t0;
+    const bool pred = kind ? (d > c) : (d >= c);
+    if (pred) hi = mid; else lo = mid + 1;
+  }
+  return lo;
+}
+
+struct TfSums {  // sums over a sample range in epoch coordinates
+  double n, s1, s2, d1, d2, xd;
+};
+
+// The statistic of one trial from cumulative sums at its boundaries
+// (cumulative from the window start): c0 = -W/2, cAB..cDE, cE = W/2 (<),
+// cM = W/2 (<=).  Returns S/N; writes H.
+__device__ __forceinline__ double tf_stat(const TfSums &c0, const TfSums &cAB,
+                                          const TfSums &cBC, const TfSums &cCD,
+                                          const TfSums &cDE, const TfSums &cE,
+                                          const TfSums &cM, double W, double L, double *Hout) {
+  // mean over [-W/2, W/2] (tls.py:187)
+  const double mcnt = cM.n - c0.n;
+  const double mean = (cM.s1 - c0.s1) / mcnt;
+  // region sums
+  const double nA = cAB.n - c0.n, nB = cBC.n - cAB.n, nC = cCD.n - cBC.n,
+               nD = cDE.n - cCD.n, nE = cE.n - cDE.n;
+  const double sA = cAB.s1 - c0.s1, sB = cBC.s1 - cAB.s1, sC = cCD.s1 - cBC.s1,
+               sD = cDE.s1 - cCD.s1, sE = cE.s1 - cDE.s1;
+  const double dB = cBC.d1 - cAB.d1, dD = cDE.d1 - cCD.d1;
+  const double d2B = cBC.d2 - cAB.d2, d2D = cDE.d2 - cCD.d2;
+  const double xdB = cBC.xd - cAB.xd, xdD = cDE.xd - cCD.xd;
+  // y = x' - mean
+  const double yA = sA - mean * nA, yB = sB - mean * nB, yC = sC - mean * nC,
+               yD = sD - mean * nD, yE = sE - mean * nE;
+  const double ydB = xdB - mean * dB, ydD = xdD - mean * dD;
+  const double xAEC = yA + yE + yC, xBD = yB + yD;
+  const double denA = 4.0 * L * L * (nA - nC + nE) + W * W * (nB + nD) + 8.0 * W * (dB - dD) +
+                      16.0 * (d2B + d2D);
+  const double numB = 8.0 * L * L * xAEC - 4.0 * W * L * xBD + 16.0 * L * (ydD - ydB);
+  const double H = -numB / denA;
+  const double h = H / 2.0;
+  const double a = H / L;
+  // sum of y^2 over A..E: samples in [-W/2, W/2)
+  const double nAll = cE.n - c0.n;
+  const double s1All = cE.s1 - c0.s1, s2All = cE.s2 - c0.s2;
+  const double y2 = s2All - 2.0 * mean * s1All + mean * mean * nAll;
+  const double q4 = W / 4.0;
+  double res = y2 + 2.0 * h * (yA - yC + yE) + h * h * (nA + nC + nE);
+  res += -2.0 * a * (ydB + q4 * yB) + a * a * (d2B + 2.0 * q4 * dB + q4 * q4 * nB);
+  res += 2.0 * a * (ydD - q4 * yD) + a * a * (d2D - 2.0 * q4 * dD + q4 * q4 * nD);
+  const double nn = nA + nB + nC + nD + nE;
+  *Hout = H;
+  return -H / sqrt(res / (nn - 3.0));
+}
+
+struct TfArgs {
+  const float *t, *x;
+  const int64_t *off;
+  const int32_t *len;
+  const double *periods;
+  const int64_t *poff;
+  const int32_t *np;
+  int32_t ncurves;
+  int64_t nptot;
+  const double *widths;
+  int32_t nw, nl;
+  double dtep;
+  double *sn, *hbest;
+  int32_t *bw, *bl, *be;
+  unsigned char *ws;
+  int64_t ws_per_block;
+  int32_t nmax;
+};
+
+// LDS layout for a staged range
+struct TfLds {
+  double lu[kTfChunk];          // images (exact)
+  double px[kTfChunk + 1][5];   // exclusive prefix of x', x'^2, q, q^2, x'q
+  double scan[kTfThreads][5];
+  int misc[8];
+  double wbnd[kTfMaxNw * kTfMaxBnd][6];  // chunk-mode accumulators (W x boundary)
+  double red_sn[kTfThreads / kWave], red_h[kTfThreads / kWave];
+  int red_w[kTfThreads / kWave], red_l[kTfThreads / kWave], red_e[kTfThreads / kWave];
+};
+
+// boundary b of width w (kind, offset): 0 -> -W/2 (>=), 1 -> W/2 (>=, region
+// end), 2 -> W/2 (>, mean end), 3 + 4m + {0,1,2,3} -> tAB, tBC, tCD, tDE (>=)
+__device__ __forceinline__ void tf_bnd(double W, int b, int nl, double &c, int &kind) {
+  if (b == 0) { c = -0.5 * W; kind = 0; return; }
+  if (b == 1) { c = 0.5 * W; kind = 0; return; }
+  if (b == 2) { c = 0.5 * W; kind = 1; return; }
+  const int m = (b - 3) >> 2, r = (b - 3) & 3;
+  double tAB, tBC, tCD, tDE;
+  tf_bounds(W, m, nl, tAB, tBC, tCD, tDE);
+  c = r == 0 ? tAB : (r == 1 ? tBC : (r == 2 ? tCD : tDE));
+  kind = 0;
+}
+
+// Stage extended range [lo, lo+cnt) into LDS with prefix sums relative to r.
+__device__ void tf_stage(TfLds &L, const double *su, const float *sx, int nv, double P, int lo,
+                         int cnt, double r, double xref) {
+  const int tid = threadIdx.x;
+  const int per = (cnt + kTfThreads - 1) / kTfThreads;
+  const int a = tid * per, b = min(cnt, a + per);
+  double acc[5] = {0, 0, 0, 0, 0};
+  for (int k = a; k < b; ++k) {
+    const int j = lo + k;
+    const double img = tf_img(su, nv, P, j);
+    L.lu[k] = img;
+    const double xv = (double)sx[tf_src(nv, j)] - xref;
+    const double q = img - r;
+    acc[0] += xv; acc[1] += xv * xv; acc[2] += q; acc[3] += q * q; acc[4] += xv * q;
+  }
+#pragma unroll
+  for (int v = 0; v < 5; ++v) L.scan[tid][v] = acc[v];
+  __syncthreads();
+  if (tid == 0) {
+    double run[5] = {0, 0, 0, 0, 0};
+    for (int s = 0; s < kTfThreads; ++s) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        const double tmp = L.scan[s][v];
+        L.scan[s][v] = run[v];
+        run[v] += tmp;
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v) L.px[cnt][v] = run[v];
+  }
+  __syncthreads();
+  double run[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) run[v] = L.scan[tid][v];
+  for (int k = a; k < b; ++k) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) L.px[k][v] = run[v];
+    const int j = lo + k;
+    const double xv = (double)sx[tf_src(nv, j)] - xref;
+    const double q = L.lu[k] - r;
+    run[0] += xv; run[1] += xv * xv; run[2] += q; run[3] += q * q; run[4] += xv * q;
+  }
+  __syncthreads();
+}
+
+// cumulative sums (epoch coordinates, shift s = t0 - r) at staged index p
+__device__ __forceinline__ TfSums tf_cum(const TfLds &L, int p, double s) {
+  const double n = (double)p;
+  const double x1 = L.px[p][0], x2 = L.px[p][1], q1 = L.px[p][2], q2 = L.px[p][3],
+               xq = L.px[p][4];
+  TfSums o;
+  o.n = n;
+  o.s1 = x1;
+  o.s2 = x2;
+  o.d1 = q1 - n * s;
+  o.d2 = q2 - 2.0 * s * q1 + n * s * s;
+  o.xd = xq - s * x1;
+  return o;
+}
+
+__global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
+  __shared__ TfLds L;
+  __shared__ int s_cnt;
+  const int tid = threadIdx.x;
+  unsigned char *ws = A.ws + (size_t)blockIdx.x * (size_t)A.ws_per_block;
+  double *su = reinterpret_cast<double *>(ws);
+  float *sx = reinterpret_cast<float *>(su + A.nmax);
+  int *cursor = reinterpret_cast<int *>(sx + A.nmax + (A.nmax & 1));
+  double wmax = 0.0;
+  for (int w = 0; w < A.nw; ++w) wmax = fmax(wmax, A.widths[w]);
+  const double hwm = 0.5 * wmax;
+  const double marg = 1e-9;
+
+  for (int64_t item = blockIdx.x; item < A.nptot; item += gridDim.x) {
+    // item -> (curve, period)
+    int lo_c = 0, hi_c = A.ncurves - 1;
+    while (lo_c < hi_c) {
+      const int mid = (lo_c + hi_c + 1) >> 1;
+      if (A.poff[mid] <= item) lo_c = mid; else hi_c = mid - 1;
+    }
+    const int c = lo_c;
+    const double P = A.periods[item];
+    const float *tc = A.t + A.off[c];
+    const float *xc = A.x + A.off[c];
+    const int n = A.len[c];
+    const int nE = (int)ceil(P / A.dtep);
+
+    // ---- 1. fold and bucket-sort by phase ------------------------------
+    if (tid == 0) s_cnt = 0;
+    __syncthreads();
+    int myv = 0;
+    for (int i = tid; i < n; i += kTfThreads) myv += (tc[i] >= 0.0f) ? 1 : 0;
+    atomicAdd(&s_cnt, myv);
+    __syncthreads();
+    const int nv = s_cnt;
+    const int nb = max(1, nv / 2);
+    const double bscale = (double)nb / P;
+    for (int b = tid; b <= nb; b += kTfThreads) cursor[b] = 0;
+    // xref: first valid sample's flux (any fixed reference cancels in the statistic)
+    if (tid == 0) {
+      int i0 = 0;
+      while (i0 < n && !(tc[i0] >= 0.0f)) ++i0;
+      L.misc[0] = i0;
+    }
+    __syncthreads();
+    const double xref = (L.misc[0] < n) ? (double)xc[L.misc[0]] : 0.0;
+    for (int i = tid; i < n; i += kTfThreads) {
+      if (!(tc[i] >= 0.0f)) continue;
+      const double u = fmod((double)tc[i], P);
+      const int b = min(nb - 1, (int)(u * bscale));
+      atomicAdd(&cursor[b], 1);
+    }
+    __syncthreads();
+    {  // exclusive scan of cursor[0..nb) in place, cursor[nb] = nv
+      const int per = (nb + kTfThreads - 1) / kTfThreads;
+      const int a = tid * per, b = min(nb, a + per);
+      int s = 0;
+      for (int k = a; k < b; ++k) s += ld_agent(&cursor[k]);
+      L.misc[1] = 0;
+      __syncthreads();
+      // serial scan of per-thread sums via LDS
+      int *part = reinterpret_cast<int *>(L.scan);
+      part[tid] = s;
+      __syncthreads();
+      if (tid == 0) {
+        int run = 0;
+        for (int k = 0; k < kTfThreads; ++k) { const int v = part[k]; part[k] = run; run += v; }
+      }
+      __syncthreads();
+      int run = part[tid];
+      for (int k = a; k < b; ++k) { const int v = ld_agent(&cursor[k]); cursor[k] = run; run += v; }
+      if (tid == 0) cursor[nb] = nv;
+      __syncthreads();
+    }
+    // scatter (cursor[b] advances to the bucket end; starts recovered below)
+    for (int i = tid; i < n; i += kTfThreads) {
+      if (!(tc[i] >= 0.0f)) continue;
+      const double u = fmod((double)tc[i], P);
+      const int b = min(nb - 1, (int)(u * bscale));
+      const int pos = atomicAdd(&cursor[b], 1);
+      su[pos] = u;
+      sx[pos] = xc[i];
+    }
+    __syncthreads();
+    // after the scatter cursor[b] == end of bucket b == start of bucket b+1;
+    // sort each bucket [start, end) by (u, x) with insertion sort
+    for (int b = tid; b < nb; b += kTfThreads) {
+      const int s0 = b == 0 ? 0 : ld_agent(&cursor[b - 1]);
+      const int s1 = ld_agent(&cursor[b]);
+      for (int k = s0 + 1; k < s1; ++k) {
+        const double ku = su[k];
+        const float kx = sx[k];
+        int j = k - 1;
+        while (j >= s0 && (su[j] > ku || (su[j] == ku && sx[j] > kx))) {
+          su[j + 1] = su[j];
+          sx[j + 1] = sx[j];
+          --j;
+        }
+        su[j + 1] = ku;
+        sx[j + 1] = kx;
+      }
+    }
+    __threadfence_block();
+    __syncthreads();
+
+    // ---- 2/3. epoch tiles ---------------------------------------------
+    TfBest best{-INFINITY, NAN, -1, -1, -1};
+    const int nw = A.nw, nl = A.nl;
+    const int nbnd = 3 + 4 * nl;
+    const int ext_lo = -nv, ext_hi = 2 * nv;  // extended index range searched
+    int e0 = 0;
+    const int te_max = max(1, kTfThreads / nw);
+    int te = te_max;
+    while (e0 < nE) {
+      int e1 = min(nE, e0 + te);
+      // union range of windows of epochs [e0, e1)
+      const double tlo = (double)e0 * A.dtep, thi = (double)(e1 - 1) * A.dtep;
+      int lo = 0, hi = 0;
+      if (tid == 0) {
+        lo = tf_search_global(su, nv, P, ext_lo, ext_hi, tlo, -hwm - marg, 0);
+        hi = tf_search_global(su, nv, P, lo, ext_hi, thi, hwm + marg, 1);
+        L.misc[2] = lo;
+        L.misc[3] = hi;
+      }
+      __syncthreads();
+      lo = L.misc[2];
+      hi = L.misc[3];
+      __syncthreads();
+      const int cnt = hi - lo;
+      if (cnt > kTfChunk && e1 - e0 > 1) {  // shrink the tile
+        te = max(1, (e1 - e0) / 2);
+        continue;
+      }
+      if (cnt <= kTfChunk) {
+        // ---- single stage: thread = (epoch, W) ----
+        const double r = tlo;
+        tf_stage(L, su, sx, nv, P, lo, cnt, r, xref);
+        const int slot = tid;
+        const int ep = e0 + slot / nw, w = slot % nw;
+        if (slot < (e1 - e0) * nw) {
+          const double t0 = (double)ep * A.dtep;
+          const double s = t0 - r;
+          const double W = A.widths[w];
+          const int p0 = tf_search(L.lu, 0, cnt, t0, -0.5 * W, 0);
+          const int pE = tf_search(L.lu, p0, cnt, t0, 0.5 * W, 0);
+          const int pM = tf_search(L.lu, pE, cnt, t0, 0.5 * W, 1);
+          const TfSums c0 = tf_cum(L, p0, s), cE = tf_cum(L, pE, s), cM = tf_cum(L, pM, s);
+          const double dl = 0.45 * W / (double)nl;
+          for (int m = 0; m < nl; ++m) {
+            double tAB, tBC, tCD, tDE;
+            tf_bounds(W, m, nl, tAB, tBC, tCD, tDE);
+            const int pAB = tf_search(L.lu, p0, pE, t0, tAB, 0);
+            const int pBC = tf_search(L.lu, pAB, pE, t0, tBC, 0);
+            const int pCD = tf_search(L.lu, pBC, pE, t0, tCD, 0);
+            const int pDE = tf_search(L.lu, pCD, pE, t0, tDE, 0);
+            double H;
+            const double sn = tf_stat(c0, tf_cum(L, pAB, s), tf_cum(L, pBC, s),
+                                      tf_cum(L, pCD, s), tf_cum(L, pDE, s), cE, cM, W,
+                                      (double)m * dl, &H);
+            if (tf_better(sn, w, m, ep, best)) best = TfBest{sn, H, w, m, ep};
+          }
+        }
+        __syncthreads();
+      } else {
+        // ---- chunked sweep of one epoch's window ----
+        const double t0 = (double)e0 * A.dtep;
+        const int tot = nw * nbnd;
+        for (int k = tid; k < tot; k += kTfThreads)
+          for (int v = 0; v < 6; ++v) L.wbnd[k][v] = 0.0;
+        __syncthreads();
+        for (int ca = lo; ca < hi; ca += kTfChunk) {
+          const int cc = min(kTfChunk, hi - ca);
+          tf_stage(L, su, sx, nv, P, ca, cc, t0, xref);
+          for (int k = tid; k < tot; k += kTfThreads) {
+            const int w = k / nbnd, b = k % nbnd;
+            double cval;
+            int kind;
+            tf_bnd(A.widths[w], b, nl, cval, kind);
+            const int p = tf_search(L.lu, 0, cc, t0, cval, kind);
+            const TfSums cs = tf_cum(L, p, 0.0);
+            L.wbnd[k][0] += cs.n; L.wbnd[k][1] += cs.s1; L.wbnd[k][2] += cs.s2;
+            L.wbnd[k][3] += cs.d1; L.wbnd[k][4] += cs.d2; L.wbnd[k][5] += cs.xd;
+          }
+          __syncthreads();
+        }
+        for (int k = tid; k < nw * nl; k += kTfThreads) {
+          const int w = k / nl, m = k % nl;
+          const double W = A.widths[w];
+          auto get = [&](int b) {
+            const double *v = L.wbnd[w * nbnd + b];
+            return TfSums{v[0], v[1], v[2], v[3], v[4], v[5]};
+          };
+          const double dl = 0.45 * W / (double)nl;
+          double H;
+          const double sn = tf_stat(get(0), get(3 + 4 * m), get(4 + 4 * m), get(5 + 4 * m),
+                                    get(6 + 4 * m), get(1), get(2), W, (double)m * dl, &H);
+          if (tf_better(sn, w, m, e0, best)) best = TfBest{sn, H, w, m, e0};
+        }
+        __syncthreads();
+      }
+      e0 = e1;
+      te = min(te_max, 2 * te);
+    }
+    // ---- reduce the per-thread bests (wave shuffles, then LDS) ----
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+      const double osn = __shfl_xor(best.sn, o), oh = __shfl_xor(best.h, o);
+      const int ow = __shfl_xor(best.w, o), ol = __shfl_xor(best.l, o),
+                oe = __shfl_xor(best.e, o);
+      if (ow >= 0 && (best.w < 0 || tf_better(osn, ow, ol, oe, best)))
+        best = TfBest{osn, oh, ow, ol, oe};
+    }
+    if ((tid & (kWave - 1)) == 0) {
+      const int wv = tid / kWave;
+      L.red_sn[wv] = best.sn; L.red_h[wv] = best.h;
+      L.red_w[wv] = best.w; L.red_l[wv] = best.l; L.red_e[wv] = best.e;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      TfBest b{L.red_sn[0], L.red_h[0], L.red_w[0], L.red_l[0], L.red_e[0]};
+      for (int k = 1; k < kTfThreads / kWave; ++k)
+        if (L.red_w[k] >= 0 &&
+            (b.w < 0 || tf_better(L.red_sn[k], L.red_w[k], L.red_l[k], L.red_e[k], b)))
+          b = TfBest{L.red_sn[k], L.red_h[k], L.red_w[k], L.red_l[k], L.red_e[k]};
+      A.sn[item] = b.sn;
+      if (A.hbest) A.hbest[item] = b.h;
+      A.bw[item] = b.w;
+      A.bl[item] = b.l;
+      A.be[item] = b.e;
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace gtrap
+
+extern "C" int64_t gtrap_tlsfold_ws_bytes(int32_t nmax, int32_t nblocks) {
+  if (nmax < 1) nmax = 1;
+  int64_t per = (int64_t)nmax * 8 + (int64_t)(nmax + 1) * 4 + (int64_t)(nmax / 2 + 2) * 4;
+  per = (per + 255) / 256 * 256;  // keep every block's slot 256-byte aligned
+  return per * (int64_t)(nblocks > 0 ? nblocks : 1);
+}
+
+extern "C" int gtrap_tlsfold(double *snr, double *hbest, int32_t *best_w, int32_t *best_l,
+                             int32_t *best_e, const float *t, const float *x, const int64_t *off,
+                             const int32_t *len, int32_t nmax, const double *periods,
+                             const int64_t *poff, int32_t ncurves, int64_t nptot,
+                             const double *widths, int32_t nw, int32_t nl, double dtep,
+                             void *ws, int64_t ws_bytes, int32_t nblocks, void *stream) {
+  GTRAP_REQUIRE(snr && best_w && best_l && best_e && t && x && off && len && periods && poff &&
+                    widths && ws,
+                "gtrap_tlsfold: null pointer");
+  GTRAP_REQUIRE(ncurves >= 1 && nptot >= 0, "gtrap_tlsfold: bad counts");
+  GTRAP_REQUIRE(nw >= 1 && nw <= gtrap::kTfMaxNw, "gtrap_tlsfold: nw must be 1..%d",
+                gtrap::kTfMaxNw);
+  GTRAP_REQUIRE(nl >= 1 && nl <= gtrap::kTfMaxNl, "gtrap_tlsfold: nl must be 1..%d",
+                gtrap::kTfMaxNl);
+  GTRAP_REQUIRE(dtep > 0.0, "gtrap_tlsfold: dtep must be positive");
+  GTRAP_REQUIRE(nmax >= 1, "gtrap_tlsfold: nmax must be >= 1");
+  GTRAP_REQUIRE(nblocks >= 1, "gtrap_tlsfold: nblocks must be >= 1");
+  GTRAP_REQUIRE(ws_bytes >= gtrap_tlsfold_ws_bytes(nmax, nblocks),
+                "gtrap_tlsfold: workspace too small");
+  if (nptot == 0) return GTRAP_OK;
+  gtrap::TfArgs A{t, x, off, len, periods, poff, nullptr, ncurves, nptot, widths, nw, nl, dtep,
+                  snr, hbest, best_w, best_l, best_e, (unsigned char *)ws,
+                  gtrap_tlsfold_ws_bytes(nmax, 1), nmax};
+  const int64_t g = nptot < nblocks ? nptot : nblocks;
+  hipLaunchKernelGGL(gtrap::tlsfold_kernel, dim3((unsigned)g), dim3(gtrap::kTfThreads), 0,
+                     gtrap::as_stream(stream), A);
+  return gtrap::check_launch("tlsfold_kernel");
+}

@@ -4,8 +4,8 @@
  * Every entry point takes plain device pointers (allocated by the caller), plain
  * sizes and a hipStream_t (passed as void*; NULL = the legacy default stream),
  * launches asynchronously and returns a status code.  Nothing throws across the
- * ABI.  Outputs are written in place; no entry point allocates device memory
- * except gtrap_tlsfold_plan_bytes-sized workspaces the caller passes in.
+ * ABI.  Outputs are written in place; no entry point allocates device memory:
+ * scratch space (gtrap_tlsfold) is a workspace the caller passes in.
  *
  * Image layout ("interleaved"): a batch of nq light curves of n samples each is
  * stored C-order as an (n, nq) float32 array, element (i, q) at i*nq + q.  This
@@ -106,6 +106,50 @@ int gtrap_blsstats(const float *blsp, const uint32_t *phase1, const uint32_t *ph
                    uint32_t nf, uint32_t nq, uint32_t patch, double df, double fmin,
                    double *pest, double *sde, uint32_t *phasebest1,
                    uint32_t *phasebest2, float *maxval, void *stream);
+
+/* ---- single-transit-epoch trapezoid least squares (gtls) ------------------ */
+#define GTRAP_GTLS_NORMAL   0  /* include/gtls_simple.h   */
+#define GTRAP_GTLS_ABSOLUTE 1  /* include/gtls_absolute.h */
+
+/*
+ * Replaces the gtls kernel of include/gtls_simple.h:1-406 and
+ * include/gtls_absolute.h (module factory gtrap/gtls.py:11-27, FILLVAL -1000,
+ * FILFACCRIT 0.75).  Epoch k < nt, curve q < nq, nw widths W = wmax - dw*iw
+ * (the reference's blockDim.x, a power of two), nl ingress steps, scoop nsc.
+ * Outputs (interleaved, index k*nq + q): tlssn, tlsw, tlst0, tlsl, tlshmax.
+ * t0min and dt are accepted and unused, as in the reference.
+ */
+int gtrap_gtls(float *tlssn, float *tlsw, float *tlst0, float *tlsl, float *tlshmax,
+               const float *lc, const float *tu, int32_t nt, int32_t nq, int32_t nw,
+               int32_t nl, int32_t nsc, float t0min, float dt, float wmax, float dw,
+               float deltat, int mode, void *stream);
+
+/* ---- periodic trapezoid least squares (tls_fold) --------------------------- */
+
+/*
+ * Periodic extension of the trapezoid statistic of gtrap/tls.py:173-231
+ * (get_chi2Hmax with divx=True, tlsfit's S/N = -H/sqrt(res/(n-3)) and
+ * nanargmax over the grid).  For every trial period P (float64, days) of a
+ * curve: u = fmod(t, P), epochs e < ceil(P/dtep) at t0 = e*dtep, offset
+ * d = u - t0 wrapped by +-P to [-P/2, P/2); widths W = widths[w] and
+ * L = m*(0.45*W/nl), m < nl (the L grid of include/gtls_simple.h:69,167).
+ * Samples with t < 0 are masked.  Best trial per period = first maximum in
+ * (w, m, e) order, NaN skipped.
+ *
+ * Ragged batch: curve c has len[c] samples at offset off[c] of the flat t, x
+ * arrays (nmax >= every len[c]); its periods are periods[poff[c] .. poff[c+1])
+ * with poff[ncurves] = nptot.  Outputs per period (index poff[c] + p): best
+ * S/N, its H, and the w, m, e indices (-1 when every trial is NaN).
+ * ws: caller workspace of at least gtrap_tlsfold_ws_bytes(nmax, nblocks)
+ * bytes; nblocks = persistent workgroups (execution parameter).
+ */
+int64_t gtrap_tlsfold_ws_bytes(int32_t nmax, int32_t nblocks);
+int gtrap_tlsfold(double *snr, double *hbest, int32_t *best_w, int32_t *best_l,
+                  int32_t *best_e, const float *t, const float *x, const int64_t *off,
+                  const int32_t *len, int32_t nmax, const double *periods,
+                  const int64_t *poff, int32_t ncurves, int64_t nptot,
+                  const double *widths, int32_t nw, int32_t nl, double dtep,
+                  void *ws, int64_t ws_bytes, int32_t nblocks, void *stream);
 
 #ifdef __cplusplus
 }

@@ -157,3 +157,353 @@ void orc_geebls(float *blsp, uint32_t *phase1, uint32_t *phase2, const float *lc
   }
   free(Y); free(Cn); free(yf);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Periodic trapezoid least squares (the fold extension of gtrap/tls.py).   */
+/*                                                                          */
+/* Folding: u = fmod(t, P) in float64 (exact); epoch e has t0 = e*dtep,      */
+/* e < ceil(P/dtep).  Offset of a sample: d = u - t0, replaced by            */
+/* (u - P) - t0 when d >= P/2 and by (u + P) - t0 when d < -P/2 (each        */
+/* operation rounded in float64).  Masked samples (t < 0) are skipped.      */
+/* Trial (W, L = m * (0.45*W/nl)): regions of tls.py:175-179 on d,           */
+/* A [-W/2, -W/4-L/2), B [-W/4-L/2, -W/4+L/2), C [-W/4+L/2, W/4-L/2),        */
+/* D [W/4-L/2, W/4+L/2), E [W/4+L/2, W/2); mean over [-W/2, W/2] (:187);     */
+/* denA, numB, H = -numB/denA (:200-202); res (:204-206);                   */
+/* S/N = -H / sqrt(res / (n - 3)) (tls.py:100).                             */
+/* ------------------------------------------------------------------------ */
+static double fold_offset(double u, double t0, double P) {
+  double d = u - t0;
+  if (d >= 0.5 * P) d = (u - P) - t0;
+  else if (d < -0.5 * P) d = (u + P) - t0;
+  return d;
+}
+
+double orc_tlsfold_trial(const float *t, const float *x, int n, double P, int e, double W,
+                         int m, int nl, double dtep, int divx, double *Hout, int *cnt) {
+  const double t0 = (double)e * dtep;
+  const double dl = 0.45 * W / (double)nl;
+  const double L = (double)m * dl;
+  const double hw = 0.5 * W, qw = 0.25 * W, hl = 0.5 * L;
+  const double tAB = -qw - hl, tBC = -qw + hl, tCD = qw - hl, tDE = qw + hl;
+  double msum = 0.0;
+  int mcnt = 0;
+  for (int i = 0; i < n; ++i) {
+    if (!(t[i] >= 0.0f)) continue;
+    const double d = fold_offset(fmod((double)t[i], P), t0, P);
+    if (d >= -hw && d <= hw) { msum += (double)x[i]; mcnt++; }
+  }
+  const double mean = divx ? msum / (double)mcnt : 0.0; /* divx=False: tls.py:189-191 */
+  int nA = 0, nB = 0, nC = 0, nD = 0, nE = 0;
+  double xAEC = 0, xBD = 0, xtB = 0, xtD = 0, tB = 0, tD = 0, t2BD = 0;
+  for (int i = 0; i < n; ++i) {
+    if (!(t[i] >= 0.0f)) continue;
+    const double d = fold_offset(fmod((double)t[i], P), t0, P);
+    const double y = (double)x[i] - mean;
+    if (d >= -hw && d < tAB) { nA++; xAEC += y; }
+    else if (d >= tAB && d < tBC) { nB++; xBD += y; xtB += y * d; tB += d; t2BD += d * d; }
+    else if (d >= tBC && d < tCD) { nC++; xAEC += y; }
+    else if (d >= tCD && d < tDE) { nD++; xBD += y; xtD += y * d; tD += d; t2BD += d * d; }
+    else if (d >= tDE && d < hw) { nE++; xAEC += y; }
+  }
+  const double denA = 4.0 * L * L * (double)(nA - nC + nE) + W * W * (double)(nB + nD) +
+                      8.0 * W * (tB - tD) + 16.0 * t2BD;
+  const double numB = 8.0 * L * L * xAEC - 4.0 * W * L * xBD + 16.0 * L * (xtD - xtB);
+  const double H = -numB / denA;
+  const double h = H / 2.0;
+  double res = 0.0;
+  for (int i = 0; i < n; ++i) {
+    if (!(t[i] >= 0.0f)) continue;
+    const double d = fold_offset(fmod((double)t[i], P), t0, P);
+    const double y = (double)x[i] - mean;
+    double r;
+    if (d >= -hw && d < tAB) r = y + h;
+    else if (d >= tAB && d < tBC) r = y - H / L * (d + W / 4.0);
+    else if (d >= tBC && d < tCD) r = y - h;
+    else if (d >= tCD && d < tDE) r = y + H / L * (d - W / 4.0);
+    else if (d >= tDE && d < hw) r = y + h;
+    else continue;
+    res += r * r;
+  }
+  const int nn = nA + nB + nC + nD + nE;
+  if (Hout) *Hout = H;
+  if (cnt) *cnt = nn;
+  return -H / sqrt(res / (double)(nn - 3));
+}
+
+/* Window-restricted version of the trial above: the same sums over the
+ * samples whose offsets were precomputed (d[k], y-values via x[k]). */
+static double trial_window(const double *d, const double *xv, int k, double W, int m, int nl,
+                           double *Hout) {
+  const double dl = 0.45 * W / (double)nl;
+  const double L = (double)m * dl;
+  const double hw = 0.5 * W, qw = 0.25 * W, hl = 0.5 * L;
+  const double tAB = -qw - hl, tBC = -qw + hl, tCD = qw - hl, tDE = qw + hl;
+  double msum = 0.0;
+  int mcnt = 0;
+  for (int i = 0; i < k; ++i)
+    if (d[i] >= -hw && d[i] <= hw) { msum += xv[i]; mcnt++; }
+  const double mean = msum / (double)mcnt;
+  int nA = 0, nB = 0, nC = 0, nD = 0, nE = 0;
+  double xAEC = 0, xBD = 0, xtB = 0, xtD = 0, tB = 0, tD = 0, t2BD = 0;
+  for (int i = 0; i < k; ++i) {
+    const double dd = d[i], y = xv[i] - mean;
+    if (dd >= -hw && dd < tAB) { nA++; xAEC += y; }
+    else if (dd >= tAB && dd < tBC) { nB++; xBD += y; xtB += y * dd; tB += dd; t2BD += dd * dd; }
+    else if (dd >= tBC && dd < tCD) { nC++; xAEC += y; }
+    else if (dd >= tCD && dd < tDE) { nD++; xBD += y; xtD += y * dd; tD += dd; t2BD += dd * dd; }
+    else if (dd >= tDE && dd < hw) { nE++; xAEC += y; }
+  }
+  const double denA = 4.0 * L * L * (double)(nA - nC + nE) + W * W * (double)(nB + nD) +
+                      8.0 * W * (tB - tD) + 16.0 * t2BD;
+  const double numB = 8.0 * L * L * xAEC - 4.0 * W * L * xBD + 16.0 * L * (xtD - xtB);
+  const double H = -numB / denA;
+  const double h = H / 2.0;
+  double res = 0.0;
+  for (int i = 0; i < k; ++i) {
+    const double dd = d[i], y = xv[i] - mean;
+    double r;
+    if (dd >= -hw && dd < tAB) r = y + h;
+    else if (dd >= tAB && dd < tBC) r = y - H / L * (dd + W / 4.0);
+    else if (dd >= tBC && dd < tCD) r = y - h;
+    else if (dd >= tCD && dd < tDE) r = y + H / L * (dd - W / 4.0);
+    else if (dd >= tDE && dd < hw) r = y + h;
+    else continue;
+    res += r * r;
+  }
+  const int nn = nA + nB + nC + nD + nE;
+  if (Hout) *Hout = H;
+  return -H / sqrt(res / (double)(nn - 3));
+}
+
+typedef struct { double u; double x; } ux_t;
+static int cmp_ux(const void *a, const void *b) {
+  const double ua = ((const ux_t *)a)->u, ub = ((const ux_t *)b)->u;
+  return (ua > ub) - (ua < ub);
+}
+
+static int lower_u(const ux_t *s, int k, double v) { /* first index with u >= v */
+  int lo = 0, hi = k;
+  while (lo < hi) {
+    const int mid = (lo + hi) / 2;
+    if (s[mid].u < v) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+void orc_tlsfold(const float *t, const float *x, int n, const double *periods, int np,
+                 const double *widths, int nw, int nl, double dtep, double *best_sn,
+                 int *best_w, int *best_l, int *best_e, double *best_h) {
+  double wmax = 0.0;
+  for (int w = 0; w < nw; ++w) if (widths[w] > wmax) wmax = widths[w];
+  const double hwm = 0.5 * wmax;
+  ux_t *s = (ux_t *)malloc(sizeof(ux_t) * (size_t)(n > 0 ? n : 1));
+  double *d = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  double *xv = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  for (int p = 0; p < np; ++p) {
+    const double P = periods[p];
+    int k = 0;
+    for (int i = 0; i < n; ++i) {
+      if (!(t[i] >= 0.0f)) continue;
+      s[k].u = fmod((double)t[i], P);
+      s[k].x = (double)x[i];
+      k++;
+    }
+    qsort(s, (size_t)k, sizeof(ux_t), cmp_ux);
+    const int nE = (int)ceil(P / dtep);
+    double bsn = -INFINITY, bh = NAN;
+    int bw = -1, bl = -1, be = -1;
+    for (int e = 0; e < nE; ++e) {
+      const double t0 = (double)e * dtep;
+      /* candidates: samples whose folded offset lies in [-Wmax/2, Wmax/2];
+       * the sorted order only narrows the scan (1e-9 d margin), the test
+       * itself uses the exact offset */
+      int kk = 0;
+      for (int img = -1; img <= 1; ++img) {
+        const double lo = t0 - hwm - 1e-9 - img * P, hi = t0 + hwm + 1e-9 - img * P;
+        for (int i = lower_u(s, k, lo); i < k && s[i].u <= hi; ++i) {
+          const double dd = fold_offset(s[i].u, t0, P);
+          /* the three image ranges are disjoint because Wmax < P */
+          if (dd >= -hwm && dd <= hwm) { d[kk] = dd; xv[kk] = s[i].x; kk++; }
+        }
+      }
+      for (int w = 0; w < nw; ++w) {
+        for (int m = 0; m < nl; ++m) {
+          double H;
+          const double sn = trial_window(d, xv, kk, widths[w], m, nl, &H);
+          const int earlier = (w < bw) || (w == bw && (m < bl || (m == bl && e < be)));
+          if (sn > bsn || (sn == bsn && earlier)) { bsn = sn; bw = w; bl = m; be = e; bh = H; }
+        }
+      }
+    }
+    best_sn[p] = bsn;
+    best_w[p] = bw;
+    best_l[p] = bl;
+    best_e[p] = be;
+    if (best_h) best_h[p] = bh;
+  }
+  free(s); free(d); free(xv);
+}
+
+/* ------------------------------------------------------------------------ */
+/* gtls: include/gtls_simple.h:1-406 (mode 0), include/gtls_absolute.h      */
+/* (mode 1; dl = W/4/nl, L = (m+1)dl, S/N uses |H|).  One "block" per epoch  */
+/* k and curve q; one "thread" per width W = wmax - dw*iw.  Float32 as the   */
+/* reference; expressions with double literals (FILLVAL -1000.0, 0.45, 0.5,  */
+/* 0.25, 4.0, 8.0, 16.0, W/4.0, Hmax/2.0) are evaluated in double and        */
+/* rounded to float where the reference assigns to a float.  a*b+c float     */
+/* patterns are fused (nvcc's default -fmad=true contraction); division and  */
+/* sqrt are IEEE (the reference's -use_fast_math approximations are not      */
+/* reproducible bit for bit).  Ties of the block maximum resolve to the      */
+/* lowest thread (largest W).                                               */
+/* ------------------------------------------------------------------------ */
+#define G_FILLVAL (-1000.0)
+#define G_FILFACCRIT 0.75
+
+static void gtls_thread(const float *cache, int nsc, int nl, float W, float deltat, int mode,
+                        float *res_max_out, float *Lmax_out, float *Hmax_out) {
+  const float *ct = cache + nsc;
+  const float dl = mode ? (float)((double)W / 4.0 / (double)(float)nl)
+                        : (float)((double)W * 0.45 / (double)(float)nl);   /* :69 */
+  const float tEnd = (float)(0.5 * (double)W), tStart = (float)(-0.5 * (double)W); /* :116 */
+  float mean = 0.0f, nsum = 0.0f;                                       /* :139-152 */
+  for (int m = 0; m < nsc; ++m) {
+    if (ct[m] >= tStart && ct[m] <= tEnd) {
+      mean = mean + cache[m];
+      nsum = (float)((double)nsum + 1.0);
+    }
+  }
+  if (nsum > 0) mean = mean / nsum;
+  float res_max = 0.0f, Lmax = 0.0f, Hmaxmax = 0.0f;
+  for (int mm = 0; mm < nl; ++mm) {                                     /* :162 */
+    const float L = (mode ? (float)(mm + 1) : (float)mm) * dl;
+    float xAEC = 0, xBD = 0, xtB = 0, xtD = 0, t2BD = 0, tB = 0, tD = 0;
+    int nA = 0, nB = 0, nC = 0, nD = 0, nE = 0;
+    int nSx = 0, nAx = 0, nBx = 0, nCx = 0, nDx = 0, nEx = 0;
+    const float tAB = (float)((double)(-W) * 0.25 - (double)L * 0.5);  /* :191-194 */
+    const float tBC = (float)((double)(-W) * 0.25 + (double)L * 0.5);
+    const float tCD = -tBC, tDE = -tAB;
+    for (int i = 0; i < nsc; ++i) { if (ct[i] > tStart) break; nSx++; }  /* :199-206 */
+    for (int i = 0; i < nsc - nSx; ++i) {                               /* :212-225 */
+      const float tnow = ct[i + nSx];
+      if (tnow > tAB) break;
+      if (tnow >= tStart) { nA++; xAEC = (xAEC + cache[i]) - mean; }   /* A.1 quirk */
+      nAx++;
+    }
+    for (int i = 0; i < nsc - nAx - nSx; ++i) {                         /* :231-249 */
+      const float tnow = ct[i + nSx + nAx];
+      if (tnow > tBC) break;
+      if (tnow >= tStart) {
+        nB++;
+        const float xtmp = cache[i + nSx + nAx] - mean;
+        xBD = xBD + xtmp;
+        tB = tB + tnow;
+        xtB = fmaf(xtmp, tnow, xtB);
+        t2BD = fmaf(tnow, tnow, t2BD);
+      }
+      nBx++;
+    }
+    for (int i = 0; i < nsc - nAx - nBx - nSx; ++i) {                   /* :252-268 */
+      const float tnow = ct[i + nSx + nAx + nBx];
+      if (tnow > tCD) break;
+      if (tnow >= tStart) { nC++; xAEC = (xAEC + cache[i + nSx + nAx + nBx]) - mean; }
+      nCx++;
+    }
+    for (int i = 0; i < nsc - nSx - nAx - nBx - nCx; ++i) {             /* :271-288 */
+      const float tnow = ct[i + nSx + nAx + nBx + nCx];
+      if (tnow > tDE) break;
+      if (tnow >= tStart) {
+        nD++;
+        const float xtmp = cache[i + nSx + nAx + nBx + nCx] - mean;
+        xBD = xBD + xtmp;
+        tD = tD + tnow;
+        xtD = fmaf(xtmp, tnow, xtD);
+        t2BD = fmaf(tnow, tnow, t2BD);
+      }
+      nDx++;
+    }
+    for (int i = 0; i < nsc - nAx - nBx - nCx - nDx - nSx; ++i) {       /* :292-305 */
+      const float tnow = ct[i + nSx + nAx + nBx + nCx + nDx];
+      if (tnow > tEnd) break;
+      if (tnow >= tStart) { nE++; xAEC = (xAEC + cache[i + nSx + nAx + nBx + nCx + nDx]) - mean; }
+      nEx++;
+    }
+    const float filfac = (float)(nA + nB + nC + nD + nE) / (W / deltat);  /* :308 */
+    if ((double)filfac > G_FILFACCRIT) {                                /* :311 */
+      const double dL = (double)L, dW = (double)W;
+      const float denA = (float)(4.0 * dL * dL * (double)(float)(nA - nC + nE) +
+                                 (double)(W * W * (float)(nB + nD)) + 8.0 * dW * (double)(tB - tD) +
+                                 16.0 * (double)t2BD);                  /* :314 */
+      const float numB = (float)(8.0 * dL * dL * (double)xAEC - 4.0 * dW * dL * (double)xBD +
+                                 16.0 * dL * (double)(xtD - xtB));      /* :315 */
+      const float Hmax = -numB / denA;                                   /* :316 */
+      const float Hmaxh = (float)((double)Hmax / 2.0);
+      const float HL = Hmax / L;
+      float res = 0.0f;
+      for (int i = nSx; i < nSx + nAx; ++i)
+        if (ct[i] >= tStart) { const float e = (cache[i] - mean) + Hmaxh; res = fmaf(e, e, res); }
+      for (int i = nSx + nAx; i < nSx + nAx + nBx; ++i)
+        if (ct[i] >= tStart) {
+          const float e = (float)((double)(cache[i] - mean) -
+                                  (double)HL * ((double)ct[i] + dW / 4.0));
+          res = fmaf(e, e, res);
+        }
+      for (int i = nSx + nAx + nBx; i < nSx + nAx + nBx + nCx; ++i)
+        if (ct[i] >= tStart) { const float e = (cache[i] - mean) - Hmaxh; res = fmaf(e, e, res); }
+      for (int i = nSx + nAx + nBx + nCx; i < nSx + nAx + nBx + nCx + nDx; ++i)
+        if (ct[i] >= tStart) {
+          const float e = (float)((double)(cache[i] - mean) +
+                                  (double)HL * ((double)ct[i] - dW / 4.0));
+          res = fmaf(e, e, res);
+        }
+      for (int i = nSx + nAx + nBx + nCx + nDx; i < nSx + nAx + nBx + nCx + nDx + nEx; ++i)
+        if (ct[i] >= tStart) { const float e = (cache[i] - mean) + Hmaxh; res = fmaf(e, e, res); }
+      if (res > 0.0f) {                                                  /* :359-369 */
+        const float hv = mode ? fabsf(Hmax) : Hmax;
+        const float sn = hv / sqrtf(res / (float)(nA + nB + nC + nD + nE - 3));
+        if (sn > res_max) { res_max = sn; Lmax = L; Hmaxmax = Hmax; }
+      }
+    }
+  }
+  *res_max_out = res_max;
+  *Lmax_out = Lmax;
+  *Hmax_out = Hmaxmax;
+}
+
+void orc_gtls(float *tlssn, float *tlsw, float *tlst0, float *tlsl, float *tlshmax,
+              const float *lc, const float *tu, int nt, int nq, int nw, int nl, int nsc,
+              float wmax, float dw, float deltat, int mode, int k0, int k1) {
+  float *cache = (float *)malloc(sizeof(float) * 2 * (size_t)(nsc > 0 ? nsc : 1));
+  const int nsch = nsc / 2;                                              /* :52 */
+  for (int k = k0; k < k1; ++k) {
+    for (int q = 0; q < nq; ++q) {
+      for (int i = 0; i < nsc; ++i) {                                    /* :28-65 */
+        cache[i] = 0.0f;
+        cache[i + nsc] = (float)G_FILLVAL;
+        const int s = k + i - nsch;
+        if (s >= 0 && s < nt) {
+          cache[i] = 0.0f + lc[(size_t)s * nq + q];
+          cache[i + nsc] = cache[i + nsc] + (float)((double)tu[(size_t)s * nq + q] - G_FILLVAL);
+        }
+      }
+      for (int i = 0; i < nsc; ++i) {                                    /* :122-134 */
+        if (cache[nsc + i] >= 0.0f) cache[i + nsc] = fmaf(-deltat, (float)k, cache[i + nsc]);
+        else if (cache[nsc + i] < 0.0f) { cache[i + nsc] = -1000.0f; cache[i] = 0.0f; }
+      }
+      float best = 0.0f, bL = 0.0f, bH = 0.0f, bW = 0.0f;
+      int have = 0;
+      for (int iw = 0; iw < nw; ++iw) {
+        const float W = fmaf(-dw, (float)iw, wmax);                       /* :68 */
+        float rm, Lm, Hm;
+        gtls_thread(cache, nsc, nl, W, deltat, mode, &rm, &Lm, &Hm);
+        if (!have || rm > best) { best = rm; bL = Lm; bH = Hm; bW = W; have = 1; }
+      }
+      const size_t o = (size_t)k * nq + q;                                /* :394-404 */
+      tlssn[o] = best;
+      tlst0[o] = deltat * (float)k;
+      tlsw[o] = bW;
+      tlsl[o] = bL;
+      tlshmax[o] = bH;
+    }
+  }
+  free(cache);
+}

@@ -17,3 +17,20 @@ void orc_geebls(float *blsp, uint32_t *phase1, uint32_t *phase2, const float *lc
                 uint32_t nq, uint32_t nf, uint32_t nb, uint32_t kma, uint32_t kmi,
                 uint32_t kkmi, float fmin, float df, uint32_t nthread, int simple,
                 uint32_t q0, uint32_t q1, uint32_t f0, uint32_t f1);
+
+/* Periodic trapezoid least squares: the statistic of gtrap/tls.py:44-79
+ * (get_chi2Hmax, divx=True, sigma=1) on time folded at each trial period,
+ * best over (epoch, W, L) per period as tlsfit's nanargmax (tls.py:82-110).
+ * Direct evaluation (every trial sums over its window samples). */
+void orc_tlsfold(const float *t, const float *x, int n, const double *periods, int np,
+                 const double *widths, int nw, int nl, double dtep, double *best_sn,
+                 int *best_w, int *best_l, int *best_e, double *best_h);
+/* One trial (period P, epoch e, width W, L index m); returns S/N, writes H and n. */
+double orc_tlsfold_trial(const float *t, const float *x, int n, double P, int e, double W,
+                         int m, int nl, double dtep, int divx, double *H, int *cnt);
+
+/* include/gtls_simple.h (mode 0) and include/gtls_absolute.h (mode 1), float32
+ * line-faithful; outputs at k*nq + q for epochs k in [k0, k1), all curves. */
+void orc_gtls(float *tlssn, float *tlsw, float *tlst0, float *tlsl, float *tlshmax,
+              const float *lc, const float *tu, int nt, int nq, int nw, int nl, int nsc,
+              float wmax, float dw, float deltat, int mode, int k0, int k1);

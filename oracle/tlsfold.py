@@ -1,0 +1,101 @@
+"""Periodic trapezoid least squares oracle (fold extension of gtrap/tls.py).
+
+Definition (shared with gtrap_amd/tlsfold.py and the HIP kernel):
+for each trial period P, every valid sample (t >= 0) is folded to
+u = fmod(t, P) (float64, exact); epoch e < ceil(P/dtep) sits at t0 = e*dtep;
+the sample's offset is d = u - t0, replaced by (u - P) - t0 when d >= P/2 and
+by (u + P) - t0 when d < -P/2.  For each width W and L = m*(0.45*W/nl)
+(m < nl; the reference's normal-mode grid, include/gtls_simple.h:69,167) the
+statistic of gtrap/tls.py:173-208 (get_chi2Hmax with divx=True, sigma=1) is
+evaluated on d, and S/N = -H/sqrt(res/(n-3)) (tls.py:229).  The best trial of
+a period is the first maximum in (W, L, epoch) order, NaN skipped
+(tls.py:217-231, nanargmax).
+
+* :func:`tlsfold` -- C, direct per-trial sums over the window samples.
+* :func:`trial` -- C, one trial from the raw arrays (no candidate selection).
+* :func:`trial_numpy` -- the tls.py masks evaluated with numpy on folded d.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._clib import fptr, lib
+
+
+def tlsfold(t, x, periods, widths, nl, dtep):
+    t = np.ascontiguousarray(t, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    periods = np.ascontiguousarray(periods, np.float64)
+    widths = np.ascontiguousarray(widths, np.float64)
+    np_ = len(periods)
+    sn = np.empty(np_, np.float64)
+    bw, bl, be = (np.empty(np_, np.int32) for _ in range(3))
+    h = np.empty(np_, np.float64)
+    f = lib().orc_tlsfold
+    f.restype = None
+    f.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                  C.c_int, C.c_double] + [C.c_void_p] * 5
+    f(fptr(t), fptr(x), len(t), fptr(periods), np_, fptr(widths), len(widths), int(nl),
+      float(dtep), fptr(sn), fptr(bw), fptr(bl), fptr(be), fptr(h))
+    return dict(sn=sn, w=bw, l=bl, e=be, h=h)
+
+
+def trial(t, x, P, e, W, m, nl, dtep, divx=True):
+    t = np.ascontiguousarray(t, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    f = lib().orc_tlsfold_trial
+    f.restype = C.c_double
+    f.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int,
+                  C.c_int, C.c_double, C.c_int, C.c_void_p, C.c_void_p]
+    H = np.zeros(1)
+    cnt = np.zeros(1, np.int32)
+    sn = f(fptr(t), fptr(x), len(t), float(P), int(e), float(W), int(m), int(nl), float(dtep),
+           1 if divx else 0, fptr(H), fptr(cnt))
+    return sn, H[0], int(cnt[0])
+
+
+def fold_offsets(t, P, e, dtep):
+    t = np.asarray(t)
+    ok = t >= 0
+    u = np.fmod(t[ok].astype(np.float64), P)
+    t0 = float(e) * dtep
+    d = u - t0
+    d = np.where(d >= 0.5 * P, (u - P) - t0, np.where(d < -0.5 * P, (u + P) - t0, d))
+    return d, ok
+
+
+def trial_numpy(t, x, P, e, W, m, nl, dtep, divx=True):
+    """tls.py's get_chi2Hmax masks and formulas (tls.py:173-208) on folded d."""
+    d, ok = fold_offsets(t, P, e, dtep)
+    xx = np.asarray(x)[ok].astype(np.float64)
+    L = m * (0.45 * W / nl)
+    tn = d
+    maskA = (tn >= -W / 2) * (tn < -W / 4 - L / 2)
+    maskB = (tn >= -W / 4 - L / 2) * (tn < -W / 4 + L / 2)
+    maskC = (tn < W / 4 - L / 2) * (tn >= -W / 4 + L / 2)
+    maskD = (tn < W / 4 + L / 2) * (tn >= W / 4 - L / 2)
+    maskE = (tn < W / 2) * (tn >= W / 4 + L / 2)
+    nA, nB, nC, nD, nE = (int(k.sum()) for k in (maskA, maskB, maskC, maskD, maskE))
+    n = nA + nB + nC + nD + nE
+    with np.errstate(all="ignore"):
+        xsum = np.nanmean(xx[(tn >= -W / 2) * (tn <= W / 2)]) if divx else 0.0
+        x_ = xx - xsum
+        xAEC = np.nansum(x_[maskA]) + np.nansum(x_[maskE]) + np.nansum(x_[maskC])
+        xBD = np.nansum(x_[maskB]) + np.nansum(x_[maskD])
+        xtB = np.nansum(x_[maskB] * tn[maskB])
+        xtD = np.nansum(x_[maskD] * tn[maskD])
+        tB = np.nansum(tn[maskB])
+        tD = np.nansum(tn[maskD])
+        t2BD = np.nansum(tn[maskB] * tn[maskB]) + np.nansum(tn[maskD] * tn[maskD])
+        denA = 4 * L * L * (nA - nC + nE) + W * W * (nB + nD) + 8 * W * (tB - tD) + 16 * t2BD
+        numB = 8 * L * L * xAEC - 4 * W * L * xBD + 16 * L * (xtD - xtB)
+        H = -numB / denA
+        Hh = H / 2.0
+        res = (np.nansum((x_[maskA] + Hh) ** 2) + np.nansum((x_[maskC] - Hh) ** 2)
+               + np.nansum((x_[maskE] + Hh) ** 2))
+        res = res + np.nansum((x_[maskB] - H / L * (tn[maskB] + W / 4.0)) ** 2)
+        res = res + np.nansum((x_[maskD] + H / L * (tn[maskD] - W / 4.0)) ** 2)
+        sn = -H / np.sqrt(res / (n - 3))
+    return sn, H, n

@@ -1,0 +1,86 @@
+"""GPU parity of gtls (single-transit-epoch TLS) against the float32 oracle.
+
+Kernel and restatement execute the same float32/float64 operations in the
+same order (fused a*b+c, IEEE division and square root), so every output
+must match bit for bit.
+"""
+import numpy as np
+import pytest
+
+from oracle import gtls as og
+
+pytestmark = pytest.mark.gpu
+
+
+def _image(nt=600, nq=3, seed=0, mask=0.03):
+    rng = np.random.default_rng(seed)
+    t = np.arange(nt, dtype=np.float64) * 0.0204
+    lc = np.empty((nt, nq), np.float32)
+    tu = np.empty((nt, nq), np.float32)
+    for q in range(nq):
+        x = 1.0 + 1e-3 * rng.standard_normal(nt)
+        k0 = rng.integers(100, nt - 100)
+        x[np.abs(t - t[k0]) < 0.1] += 3e-3  # flipped transit (2 - lc)
+        tt = t.copy()
+        tt[rng.random(nt) < mask] = -1000.0
+        lc[:, q] = x
+        tu[:, q] = tt
+    return lc, tu
+
+
+def _run(lc, tu, nw, nl, nsc, wmax, dw, deltat, mode):
+    import torch
+    from gtrap_amd import gtls
+    nt, nq = lc.shape
+    outs = [torch.full((nt * nq,), -7.0, dtype=torch.float32, device="cuda") for _ in range(5)]
+    gtls.gtls_device(*outs, torch.from_numpy(lc).cuda(), torch.from_numpy(tu).cuda(), nt, nq, nw,
+                     nl, nsc, wmax, dw, deltat, mode=mode)
+    torch.cuda.synchronize()
+    return [o.cpu().numpy().reshape(nt, nq) for o in outs]
+
+
+@pytest.mark.parametrize("mode", ["normal", "absolute"])
+@pytest.mark.parametrize("nw,nl", [(16, 10), (64, 20), (1, 5), (512, 4)])
+def test_gtls_bit_exact(gpu, mode, nw, nl):
+    lc, tu = _image(seed=nw + nl)
+    wmin, wmax = 0.05, 0.3
+    dw = np.float32((wmax - wmin) / max(nw - 1, 1))
+    deltat = np.float32(0.0204)
+    nsc = int(wmax / deltat + 3.0)
+    got = _run(lc, tu, nw, nl, nsc, np.float32(wmax), dw, deltat, mode)
+    want = og.gtls(lc, tu, nw, nl, nsc, np.float32(wmax), dw, deltat, mode=mode)
+    for g, name in zip(got, ("tlssn", "tlsw", "tlst0", "tlsl", "tlshmax")):
+        np.testing.assert_array_equal(g.view(np.uint32), want[name].view(np.uint32), err_msg=name)
+
+
+def test_gtls_dropin_module_kepler_shape(gpu):
+    """Launch exactly as kepler/gtls_mockkepler.py:305-312 (nw=1024 widths)."""
+    import torch
+    from gtrap_amd import driver as cuda
+    from gtrap_amd import gtls
+    lc, tu = _image(nt=300, nq=2, seed=9)
+    nt, nq = lc.shape
+    prm = gtls.gtls_params(tu[:, 0], 0.05, 0.3, 1024, 8)
+    dev_lc = cuda.mem_alloc(lc.nbytes)
+    dev_tu = cuda.mem_alloc(tu.nbytes)
+    cuda.memcpy_htod(dev_lc, lc)
+    cuda.memcpy_htod(dev_tu, tu)
+    outs = [cuda.mem_alloc(4 * nt * nq) for _ in range(5)]
+    k = gtls.gtls_module("normal").get_function("gtls")
+    k(*outs, dev_lc, dev_tu, np.int32(nt), np.int32(prm["nl"]), np.int32(prm["nsc"]),
+      np.float32(0.1), np.float32(1.0), prm["wmax"], prm["dw"], prm["deltat"],
+      block=(1024, 1, 1), grid=(nt, nq), shared=prm["shared"])
+    host = []
+    for o in outs:
+        h = np.zeros(nt * nq, np.float32)
+        cuda.memcpy_dtoh(h, o)
+        host.append(h.reshape(nt, nq))
+    want = og.gtls(lc, tu, 1024, prm["nl"], prm["nsc"], prm["wmax"], prm["dw"], prm["deltat"])
+    for g, name in zip(host, ("tlssn", "tlsw", "tlst0", "tlsl", "tlshmax")):
+        np.testing.assert_array_equal(g.view(np.uint32), want[name].view(np.uint32), err_msg=name)
+
+
+def test_gtls_rejects_non_power_of_two(gpu):
+    lc, tu = _image(nt=50, nq=1)
+    with pytest.raises(ValueError):
+        _run(lc, tu, 12, 4, 10, np.float32(0.2), np.float32(0.01), np.float32(0.0204), "normal")
