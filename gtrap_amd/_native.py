@@ -20,6 +20,7 @@ SIGNATURES: dict[str, tuple] = {
     "gtrap_version": (C.c_char_p, []),
     "gtrap_medfilt": (_int, [_vp, _vp, _u32, _u32, _u32, _int, _u32, _u32, _u32, _vp]),
     "gtrap_offset": (_int, [_vp, _vp, _u32, _u32, _vp, _vp, _vp, _vp]),
+    "gtrap_medfilt_ragged": (_int, [_vp] * 7 + [_i32, _u32, _int, _i32, _vp]),
     "gtrap_geebls": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _u32, _u32, _u32, _u32,
                             _u32, _u32, _f32, _f32, _u32, _vp]),
     "gtrap_geebls_simple": (_int, [_vp, _vp, _vp, _u32, _u32, _u32, _u32, _u32, _u32, _u32,
@@ -30,6 +31,9 @@ SIGNATURES: dict[str, tuple] = {
     "gtrap_tlsfold_ws_bytes": (_i64, [_i32, _i32]),
     "gtrap_tlsfold": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _i32,
                              _i64, _vp, _i32, _i32, _f64, _vp, _i64, _i32, _vp]),
+    "gtrap_tlsfold_items": (_int, [_vp] * 9 + [_i32] + [_vp] * 6 + [_i32, _i32, _f64, _vp, _i64,
+                                                                  _i32, _vp]),
+    "gtrap_tlsstats": (_int, [_vp] * 5 + [_i32] + [_vp] * 4),
 }
 
 _lib = None

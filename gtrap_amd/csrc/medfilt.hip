@@ -35,26 +35,19 @@ __device__ __forceinline__ void slide_sorted(float (&s)[W], float ov, float nv) 
 
 constexpr int kUnroll = 8;
 
+// Filter rows [i0, i1) of one curve whose sample i sits at xq[i*stride]
+// (n samples, zero padding outside [0, n)); the sorted window lives in
+// registers and slides one row at a time.
 template <int RMAX>
-__global__ __launch_bounds__(256) void medfilt_win_kernel(
-    const float *__restrict__ x, float *__restrict__ y, int n, int nq, int r, int mode,
-    int rows_out, int cols_out, int seg, int nseg) {
+__device__ __forceinline__ void medfilt_run(const float *__restrict__ xq, float *__restrict__ yq,
+                                            size_t stride, int n, int i0, int i1, int r,
+                                            int mode) {
   constexpr int W = 2 * RMAX + 1;
-  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= (long)nseg * cols_out) return;
-  const int q = (int)(g % cols_out);
-  const int sidx = (int)(g / cols_out);
-  const int i0 = sidx * seg;
-  const int i1 = min(i0 + seg, rows_out);
-  if (i0 >= i1) return;
-
   float s[W];
 #pragma unroll
   for (int k = 0; k < W; ++k)
     s[k] = (k < RMAX - r) ? -INFINITY : ((k > RMAX + r) ? INFINITY : 0.0f);
-
-  const float *xq = x + q;
-  auto ld = [&](int i) -> float { return (i >= 0 && i < n) ? xq[(size_t)i * nq] : 0.0f; };
+  auto ld = [&](int i) -> float { return (i >= 0 && i < n) ? xq[(size_t)i * stride] : 0.0f; };
 
   // The 2r+1 middle slots start as placeholder zeros; replace them one by one
   // with the window of row i0 (loads issued kUnroll at a time).
@@ -70,7 +63,6 @@ __global__ __launch_bounds__(256) void medfilt_win_kernel(
   // Main sweep, software-pipelined: the loads of the next kUnroll rows are in
   // flight while the current rows are filtered (the sweep is otherwise bound
   // by one dependent global-load latency per row).
-  float *yq = y + q;
   float cin[kUnroll], cout_[kUnroll], cc[kUnroll];
 #pragma unroll
   for (int u = 0; u < kUnroll; ++u) {
@@ -92,7 +84,7 @@ __global__ __launch_bounds__(256) void medfilt_win_kernel(
     for (int u = 0; u < kUnroll; ++u) {
       if (i + u < i1) {
         const float med = s[RMAX];
-        yq[(size_t)(i + u) * nq] = mode ? cc[u] / med : cc[u] - med;
+        yq[(size_t)(i + u) * stride] = mode ? cc[u] / med : cc[u] - med;
         slide_sorted<W>(s, cout_[u], cin[u]);
       }
     }
@@ -103,6 +95,39 @@ __global__ __launch_bounds__(256) void medfilt_win_kernel(
       cout_[u] = nout[u];
     }
   }
+}
+
+template <int RMAX>
+__global__ __launch_bounds__(256) void medfilt_win_kernel(
+    const float *__restrict__ x, float *__restrict__ y, int n, int nq, int r, int mode,
+    int rows_out, int cols_out, int seg, int nseg) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (long)nseg * cols_out) return;
+  const int q = (int)(g % cols_out);
+  const int sidx = (int)(g / cols_out);
+  const int i0 = sidx * seg;
+  const int i1 = min(i0 + seg, rows_out);
+  if (i0 >= i1) return;
+  medfilt_run<RMAX>(x + q, y + q, (size_t)nq, n, i0, i1, r, mode);
+}
+
+// Ragged batch of contiguous curves (serving): lane g filters segment g,
+// rows [seg_start[g], seg_start[g]+seg) of curve seg_curve[g]; the segment
+// count is read from device memory so the launch shape is fixed.
+template <int RMAX>
+__global__ __launch_bounds__(256) void medfilt_ragged_kernel(
+    const float *__restrict__ x, float *__restrict__ y, const int64_t *__restrict__ off,
+    const int32_t *__restrict__ len, const int32_t *__restrict__ seg_curve,
+    const int32_t *__restrict__ seg_start, const int32_t *__restrict__ nseg_dev, int r, int mode,
+    int seg) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= *nseg_dev) return;
+  const int c = seg_curve[g];
+  const int n = len[c];
+  const int i0 = seg_start[g];
+  const int i1 = min(i0 + seg, n);
+  if (i0 >= i1) return;
+  medfilt_run<RMAX>(x + off[c], y + off[c], 1, n, i0, i1, r, mode);
 }
 
 // Order-preserving map float -> uint32 (total order, -0 < +0).
@@ -240,6 +265,51 @@ extern "C" int gtrap_medfilt(const float *in, float *out, uint32_t n, uint32_t n
                              void *stream) {
   return gtrap::medfilt_launch(in, out, n, nq, r, mode, rows_out, cols_out, seg_len,
                                gtrap::as_stream(stream));
+}
+
+namespace gtrap {
+
+template <int RMAX>
+static void launch_ragged(const float *x, float *y, const int64_t *off, const int32_t *len,
+                          const int32_t *sc, const int32_t *ss, const int32_t *nseg, int max_segs,
+                          int r, int mode, int seg, hipStream_t st) {
+  hipLaunchKernelGGL((medfilt_ragged_kernel<RMAX>), dim3((max_segs + 255) / 256), dim3(256), 0,
+                     st, x, y, off, len, sc, ss, nseg, r, mode, seg);
+}
+
+int medfilt_ragged_launch(const float *x, float *y, const int64_t *off, const int32_t *len,
+                          const int32_t *sc, const int32_t *ss, const int32_t *nseg,
+                          int32_t max_segs, uint32_t r, int mode, int32_t seg,
+                          hipStream_t st) {
+  GTRAP_REQUIRE(x && y && off && len && sc && ss && nseg, "gtrap_medfilt_ragged: null pointer");
+  GTRAP_REQUIRE(mode == 0 || mode == 1, "gtrap_medfilt_ragged: mode must be 0 or 1");
+  GTRAP_REQUIRE(r <= 64, "gtrap_medfilt_ragged: r must be <= 64");
+  GTRAP_REQUIRE(seg >= 1 && max_segs >= 0, "gtrap_medfilt_ragged: bad segment shape");
+  if (max_segs == 0) return GTRAP_OK;
+  int rmax = 64;
+  for (int b : kBuckets)
+    if ((int)r <= b) { rmax = b; break; }
+  switch (rmax) {
+    case 4: launch_ragged<4>(x, y, off, len, sc, ss, nseg, max_segs, r, mode, seg, st); break;
+    case 8: launch_ragged<8>(x, y, off, len, sc, ss, nseg, max_segs, r, mode, seg, st); break;
+    case 16: launch_ragged<16>(x, y, off, len, sc, ss, nseg, max_segs, r, mode, seg, st); break;
+    case 24: launch_ragged<24>(x, y, off, len, sc, ss, nseg, max_segs, r, mode, seg, st); break;
+    case 32: launch_ragged<32>(x, y, off, len, sc, ss, nseg, max_segs, r, mode, seg, st); break;
+    case 48: launch_ragged<48>(x, y, off, len, sc, ss, nseg, max_segs, r, mode, seg, st); break;
+    default: launch_ragged<64>(x, y, off, len, sc, ss, nseg, max_segs, r, mode, seg, st); break;
+  }
+  return check_launch("medfilt_ragged_kernel");
+}
+
+}  // namespace gtrap
+
+extern "C" int gtrap_medfilt_ragged(const float *x, float *y, const int64_t *off,
+                                    const int32_t *len, const int32_t *seg_curve,
+                                    const int32_t *seg_start, const int32_t *nseg_dev,
+                                    int32_t max_segs, uint32_t r, int mode, int32_t seg_len,
+                                    void *stream) {
+  return gtrap::medfilt_ragged_launch(x, y, off, len, seg_curve, seg_start, nseg_dev, max_segs,
+                                      r, mode, seg_len, gtrap::as_stream(stream));
 }
 
 extern "C" int gtrap_offset(const float *img, const float *tu, uint32_t n, uint32_t nq,

@@ -164,6 +164,13 @@ struct TfArgs {
   unsigned char *ws;
   int64_t ws_per_block;
   int32_t nmax;
+  // item-list mode (serving): item k is period item_pidx[k] of curve slot
+  // item_curve[k]; the count is read from *nitems_dev.  counter != null
+  // hands items out dynamically (atomic ticket), else grid-stride.
+  const int32_t *item_curve;
+  const int64_t *item_pidx;
+  const int32_t *nitems_dev;
+  int32_t *counter;
 };
 
 // LDS layout for a staged range
@@ -264,14 +271,32 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
   const double hwm = 0.5 * wmax;
   const double marg = 1e-9;
 
-  for (int64_t item = blockIdx.x; item < A.nptot; item += gridDim.x) {
-    // item -> (curve, period)
-    int lo_c = 0, hi_c = A.ncurves - 1;
-    while (lo_c < hi_c) {
-      const int mid = (lo_c + hi_c + 1) >> 1;
-      if (A.poff[mid] <= item) lo_c = mid; else hi_c = mid - 1;
+  __shared__ int64_t s_item;
+  const int64_t nitems = A.nitems_dev ? (int64_t)*A.nitems_dev : A.nptot;
+  for (int64_t it = blockIdx.x;; it += gridDim.x) {
+    int64_t slot = it;
+    if (A.counter) {
+      if (tid == 0) s_item = (int64_t)atomicAdd(A.counter, 1);
+      __syncthreads();
+      slot = s_item;
+      __syncthreads();
     }
-    const int c = lo_c;
+    if (slot >= nitems) break;
+    // slot -> (curve, period index)
+    int c;
+    int64_t item;
+    if (A.item_curve) {
+      c = A.item_curve[slot];
+      item = A.item_pidx[slot];
+    } else {
+      item = slot;
+      int lo_c = 0, hi_c = A.ncurves - 1;
+      while (lo_c < hi_c) {
+        const int mid = (lo_c + hi_c + 1) >> 1;
+        if (A.poff[mid] <= item) lo_c = mid; else hi_c = mid - 1;
+      }
+      c = lo_c;
+    }
     const double P = A.periods[item];
     const float *tc = A.t + A.off[c];
     const float *xc = A.x + A.off[c];
@@ -515,9 +540,108 @@ extern "C" int gtrap_tlsfold(double *snr, double *hbest, int32_t *best_w, int32_
   if (nptot == 0) return GTRAP_OK;
   gtrap::TfArgs A{t, x, off, len, periods, poff, nullptr, ncurves, nptot, widths, nw, nl, dtep,
                   snr, hbest, best_w, best_l, best_e, (unsigned char *)ws,
-                  gtrap_tlsfold_ws_bytes(nmax, 1), nmax};
+                  gtrap_tlsfold_ws_bytes(nmax, 1), nmax, nullptr, nullptr, nullptr, nullptr};
   const int64_t g = nptot < nblocks ? nptot : nblocks;
   hipLaunchKernelGGL(gtrap::tlsfold_kernel, dim3((unsigned)g), dim3(gtrap::kTfThreads), 0,
                      gtrap::as_stream(stream), A);
   return gtrap::check_launch("tlsfold_kernel");
+}
+
+extern "C" int gtrap_tlsfold_items(double *snr, double *hbest, int32_t *best_w, int32_t *best_l,
+                                   int32_t *best_e, const float *t, const float *x,
+                                   const int64_t *off, const int32_t *len, int32_t nmax,
+                                   const double *periods, const int32_t *item_curve,
+                                   const int64_t *item_pidx, const int32_t *nitems_dev,
+                                   int32_t *counter, const double *widths, int32_t nw,
+                                   int32_t nl, double dtep, void *ws, int64_t ws_bytes,
+                                   int32_t nblocks, void *stream) {
+  GTRAP_REQUIRE(snr && best_w && best_l && best_e && t && x && off && len && periods &&
+                    item_curve && item_pidx && nitems_dev && counter && widths && ws,
+                "gtrap_tlsfold_items: null pointer");
+  GTRAP_REQUIRE(nw >= 1 && nw <= gtrap::kTfMaxNw, "gtrap_tlsfold_items: nw must be 1..%d",
+                gtrap::kTfMaxNw);
+  GTRAP_REQUIRE(nl >= 1 && nl <= gtrap::kTfMaxNl, "gtrap_tlsfold_items: nl must be 1..%d",
+                gtrap::kTfMaxNl);
+  GTRAP_REQUIRE(dtep > 0.0 && nmax >= 1 && nblocks >= 1, "gtrap_tlsfold_items: bad scalars");
+  GTRAP_REQUIRE(ws_bytes >= gtrap_tlsfold_ws_bytes(nmax, nblocks),
+                "gtrap_tlsfold_items: workspace too small");
+  gtrap::TfArgs A{t, x, off, len, periods, nullptr, nullptr, 0, 0, widths, nw, nl, dtep,
+                  snr, hbest, best_w, best_l, best_e, (unsigned char *)ws,
+                  gtrap_tlsfold_ws_bytes(nmax, 1), nmax, item_curve, item_pidx, nitems_dev,
+                  counter};
+  hipLaunchKernelGGL(gtrap::tlsfold_kernel, dim3((unsigned)nblocks), dim3(gtrap::kTfThreads), 0,
+                     gtrap::as_stream(stream), A);
+  return gtrap::check_launch("tlsfold_kernel");
+}
+
+namespace gtrap {
+
+// Periodogram statistics of one curve's periods [poff[c], poff[c]+np[c]):
+// best = first maximum of S/N (NaN and -inf skipped), mean and standard
+// deviation over the finite S/N values, SDE = (best - mean)/std (the BLS
+// statistic of gtrap/getstat.py:114-118 applied to the TLS periodogram).
+__global__ __launch_bounds__(256) void tlsstats_kernel(const double *__restrict__ sn,
+                                                       const int64_t *__restrict__ poff,
+                                                       const int32_t *__restrict__ np_,
+                                                       const int32_t *__restrict__ curves,
+                                                       const int32_t *__restrict__ ncurves_dev,
+                                                       int64_t *best_pidx, double *best_sn,
+                                                       double *sde) {
+  const int k = blockIdx.x;
+  if (k >= *ncurves_dev) return;
+  const int c = curves[k];
+  const int64_t a = poff[c], m = np_[c];
+  double bs = -INFINITY, s1 = 0.0, s2 = 0.0, cnt = 0.0;
+  int64_t bi = -1;
+  for (int64_t i = threadIdx.x; i < m; i += blockDim.x) {
+    const double v = sn[a + i];
+    if (isfinite(v)) {
+      s1 += v;
+      s2 += v * v;
+      cnt += 1.0;
+      if (v > bs) { bs = v; bi = i; }
+    }
+  }
+  __shared__ double r_bs[256], r_s1[256], r_s2[256], r_c[256];
+  __shared__ int64_t r_bi[256];
+  const int t = threadIdx.x;
+  r_bs[t] = bs; r_bi[t] = bi; r_s1[t] = s1; r_s2[t] = s2; r_c[t] = cnt;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) {
+      const double ob = r_bs[t + s];
+      const int64_t oi = r_bi[t + s];
+      if (oi >= 0 && (r_bi[t] < 0 || ob > r_bs[t] || (ob == r_bs[t] && oi < r_bi[t]))) {
+        r_bs[t] = ob;
+        r_bi[t] = oi;
+      }
+      r_s1[t] += r_s1[t + s];
+      r_s2[t] += r_s2[t + s];
+      r_c[t] += r_c[t + s];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    const double mean = r_s1[0] / r_c[0];
+    const double var = r_s2[0] / r_c[0] - mean * mean;
+    best_pidx[c] = r_bi[0] >= 0 ? a + r_bi[0] : -1;
+    best_sn[c] = r_bs[0];
+    sde[c] = (r_bs[0] - mean) / sqrt(var);
+  }
+}
+
+}  // namespace gtrap
+
+extern "C" int gtrap_tlsstats(const double *sn, const int64_t *poff, const int32_t *np_,
+                              const int32_t *curves, const int32_t *ncurves_dev,
+                              int32_t max_curves, int64_t *best_pidx, double *best_sn,
+                              double *sde, void *stream) {
+  GTRAP_REQUIRE(sn && poff && np_ && curves && ncurves_dev && best_pidx && best_sn && sde,
+                "gtrap_tlsstats: null pointer");
+  GTRAP_REQUIRE(max_curves >= 0, "gtrap_tlsstats: bad max_curves");
+  if (max_curves == 0) return GTRAP_OK;
+  hipLaunchKernelGGL(gtrap::tlsstats_kernel, dim3((unsigned)max_curves), dim3(256), 0,
+                     gtrap::as_stream(stream), sn, poff, np_, curves, ncurves_dev, best_pidx,
+                     best_sn, sde);
+  return gtrap::check_launch("tlsstats_kernel");
 }

@@ -151,3 +151,39 @@ def config3(nq: int = 4096, n: int = 65536, seed: int = 1003, device=None):
     out = rng.random((n, nq)) < 1e-3
     f[out] += 1e-2 * np.where(rng.random(out.sum()) < 0.5, -1.0, 1.0)
     return f.astype(np.float32)
+
+
+def config4_requests(count: int = 200, seed: int = 1004, rate: float = 2000.0,
+                     nlo: int = 2048, nhi: int = 70000, period_stride: int = 1):
+    """Serving trace: Poisson arrivals (mean ``rate`` req/s), lengths
+    log-uniform nlo..nhi at 0.0204 d cadence, flux = 1 + slow trend + noise
+    1e-3, trapezoid transits (depth U(1e-3, 5e-3), P U(0.5, baseline/3),
+    T14 U(0.05, 0.3), ingress T14/10) in 30% of the requests; periods
+    geomspace(0.5, baseline/3, m) with m 500..4000 log-linear in n."""
+    from .serve import Request
+    from .tlsfold import period_grid
+    rng = np.random.default_rng(seed)
+    arr = np.cumsum(rng.exponential(1.0 / rate, count))
+    out, truth = [], []
+    for i in range(count):
+        n = int(round(np.exp(rng.uniform(np.log(nlo), np.log(nhi)))))
+        t = np.arange(n) * CADENCE
+        base = n * CADENCE
+        flux = 1.0 + 2e-3 * np.sin(2 * np.pi * t / rng.uniform(5, 50) + rng.uniform(0, 6.3))
+        flux += rng.normal(0, 1e-3, n)
+        tr = None
+        if rng.random() < 0.3:
+            P = rng.uniform(0.5, base / 3)
+            t14 = rng.uniform(0.05, 0.3)
+            tr = dict(P=P, epoch=rng.uniform(0, P), depth=rng.uniform(1e-3, 5e-3), t14=t14)
+            flux -= trapezoid_dip(t, tr["epoch"], P, tr["depth"], t14, t14 / 10)
+        per = period_grid(base, n=n)[::period_stride]
+        out.append(Request(i, t.astype(np.float32), flux.astype(np.float32), per,
+                           arrival=float(arr[i])))
+        truth.append(tr)
+    return out, truth
+
+
+def trial_count(req, nw: int = 10, nl: int = 10, dtep: float = CADENCE) -> int:
+    """(epoch, W, L) trial evaluations of one request."""
+    return int(np.sum(np.ceil(np.asarray(req.periods) / dtep)) * nw * nl)

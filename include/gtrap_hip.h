@@ -53,6 +53,19 @@ int gtrap_medfilt(const float *in, float *out, uint32_t n, uint32_t nq, uint32_t
                   void *stream);
 
 /*
+ * Ragged-batch form of gtrap_medfilt for the serving loop (no reference
+ * counterpart: the reference detrends one rectangular image per call).  Curve
+ * c holds len[c] contiguous samples at x + off[c]; segment g filters rows
+ * [seg_start[g], seg_start[g] + seg_len) of curve seg_curve[g] into y + off[c].
+ * The segment count is read from *nseg_dev (<= max_segs) so the launch shape
+ * is fixed and the call can be captured in a HIP graph.  r <= 64.
+ */
+int gtrap_medfilt_ragged(const float *x, float *y, const int64_t *off, const int32_t *len,
+                         const int32_t *seg_curve, const int32_t *seg_start,
+                         const int32_t *nseg_dev, int32_t max_segs, uint32_t r, int mode,
+                         int32_t seg_len, void *stream);
+
+/*
  * Replaces the host-side offset of gtrap/gfilter.py:94-101 (get_offset):
  *   offset[q] = (sum_i [tu[i,q] >= 0] * img[i,q]) / n      (division by n, not ntrue)
  * accumulated in float64 on the device; also writes ntrue[q] = #{i : tu[i,q] >= 0}
@@ -150,6 +163,33 @@ int gtrap_tlsfold(double *snr, double *hbest, int32_t *best_w, int32_t *best_l,
                   const int64_t *poff, int32_t ncurves, int64_t nptot,
                   const double *widths, int32_t nw, int32_t nl, double dtep,
                   void *ws, int64_t ws_bytes, int32_t nblocks, void *stream);
+
+/*
+ * Item-list form of gtrap_tlsfold for the serving loop (no reference
+ * counterpart).  Work item k scores period item_pidx[k] (index into periods
+ * and the outputs) of curve slot item_curve[k] (off/len tables); the item
+ * count is *nitems_dev and items are handed out through the atomic ticket
+ * *counter, which the caller zeroes before the launch.  nblocks persistent
+ * workgroups; launch shape fixed (HIP-graph capturable).
+ */
+int gtrap_tlsfold_items(double *snr, double *hbest, int32_t *best_w, int32_t *best_l,
+                        int32_t *best_e, const float *t, const float *x, const int64_t *off,
+                        const int32_t *len, int32_t nmax, const double *periods,
+                        const int32_t *item_curve, const int64_t *item_pidx,
+                        const int32_t *nitems_dev, int32_t *counter, const double *widths,
+                        int32_t nw, int32_t nl, double dtep, void *ws, int64_t ws_bytes,
+                        int32_t nblocks, void *stream);
+
+/*
+ * Periodogram statistics of the tls_fold output, the TLS counterpart of
+ * gtrap/getstat.py:100-123: for curve slots curves[0 .. *ncurves_dev) (at most
+ * max_curves), over periods [poff[c], poff[c] + np[c]): best_pidx[c] = index
+ * of the first maximum S/N (NaN/-inf skipped, -1 if none), best_sn[c], and
+ * sde[c] = (best - mean)/std over the finite S/N values.  float64.
+ */
+int gtrap_tlsstats(const double *sn, const int64_t *poff, const int32_t *np_,
+                   const int32_t *curves, const int32_t *ncurves_dev, int32_t max_curves,
+                   int64_t *best_pidx, double *best_sn, double *sde, void *stream);
 
 #ifdef __cplusplus
 }
