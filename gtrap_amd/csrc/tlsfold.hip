@@ -77,19 +77,8 @@ __device__ __forceinline__ int tf_src(int nv, int j) {
   return j < 0 ? j + nv : (j >= nv ? j - nv : j);
 }
 
-// first extended index j in [lo, hi) with img(j) - t0 >= c (kind 0) or > c (kind 1)
-__device__ int tf_search_global(const double *su, int nv, double P, int lo, int hi, double t0,
-                                double c, int kind) {
-  while (lo < hi) {
-    const int mid = lo + ((hi - lo) >> 1);
-    const double d = tf_img(su, nv, P, mid) - t0;
-    const bool pred = kind ? (d > c) : (d >= c);
-    if (pred) hi = mid; else lo = mid + 1;
-  }
-  return lo;
-}
-
-// same over an LDS array of images
+// first index k in [lo, hi) of the LDS images with lu[k] - t0 >= c (kind 0)
+// or > c (kind 1)
 __device__ __forceinline__ int tf_search(const double *lu, int lo, int hi, double t0, double c,
                                          int kind) {
   while (lo < hi) {
@@ -99,6 +88,29 @@ __device__ __forceinline__ int tf_search(const double *lu, int lo, int hi, doubl
     if (pred) hi = mid; else lo = mid + 1;
   }
   return lo;
+}
+
+// Conservative extended index bound of value v from the bucket offsets
+// (end[b] = cursor[b] after the scatter): upper=0 -> an index <= the first
+// image >= v; upper=1 -> an index >= the last image <= v, plus one.
+__device__ __forceinline__ int tf_bucket_bound(const int *cursor, int nb, int nv, double P,
+                                               double bscale, double v, int upper) {
+  int sh = 0;
+  double w = v;
+  if (v < 0.0) { sh = -1; w = v + P; }
+  else if (v >= P) { sh = 1; w = v - P; }
+  int bb = (int)(w * bscale);
+  bb = bb < 0 ? 0 : (bb > nb - 1 ? nb - 1 : bb);
+  int idx;
+  if (!upper) {
+    const int b0 = bb - 1;
+    idx = b0 < 0 ? 0 : ld_agent(&cursor[b0]);  // start of bucket bb-1
+  } else {
+    const int b1 = bb + 1 > nb - 1 ? nb - 1 : bb + 1;
+    idx = ld_agent(&cursor[b1]);  // end of bucket bb+1
+  }
+  idx += sh * nv;
+  return idx < -nv ? -nv : (idx > 2 * nv ? 2 * nv : idx);
 }
 
 struct TfSums {  // sums over a sample range in epoch coordinates
@@ -177,7 +189,7 @@ struct TfArgs {
 struct TfLds {
   double lu[kTfChunk];          // images (exact)
   double px[kTfChunk + 1][5];   // exclusive prefix of x', x'^2, q, q^2, x'q
-  double scan[kTfThreads][5];
+  double scan[kTfThreads / kWave][5];
   int misc[8];
   double wbnd[kTfMaxNw * kTfMaxBnd][6];  // chunk-mode accumulators (W x boundary)
   double red_sn[kTfThreads / kWave], red_h[kTfThreads / kWave];
@@ -197,48 +209,72 @@ __device__ __forceinline__ void tf_bnd(double W, int b, int nl, double &c, int &
   kind = 0;
 }
 
-// Stage extended range [lo, lo+cnt) into LDS with prefix sums relative to r.
+// Stage extended range [lo, lo+cnt) (cnt <= kTfChunk) into LDS with
+// exclusive prefix sums of (x', x'^2, q, q^2, x'q), q = image - r.  Thread t
+// owns kPer consecutive elements; the scan is a wave shuffle scan plus a
+// cross-wave fix-up.
+constexpr int kPer = kTfChunk / kTfThreads;
 __device__ void tf_stage(TfLds &L, const double *su, const float *sx, int nv, double P, int lo,
                          int cnt, double r, double xref) {
-  const int tid = threadIdx.x;
-  const int per = (cnt + kTfThreads - 1) / kTfThreads;
-  const int a = tid * per, b = min(cnt, a + per);
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+  double xs[kPer], qs[kPer];
   double acc[5] = {0, 0, 0, 0, 0};
-  for (int k = a; k < b; ++k) {
-    const int j = lo + k;
-    const double img = tf_img(su, nv, P, j);
-    L.lu[k] = img;
-    const double xv = (double)sx[tf_src(nv, j)] - xref;
-    const double q = img - r;
-    acc[0] += xv; acc[1] += xv * xv; acc[2] += q; acc[3] += q * q; acc[4] += xv * q;
-  }
 #pragma unroll
-  for (int v = 0; v < 5; ++v) L.scan[tid][v] = acc[v];
-  __syncthreads();
-  if (tid == 0) {
-    double run[5] = {0, 0, 0, 0, 0};
-    for (int s = 0; s < kTfThreads; ++s) {
-#pragma unroll
-      for (int v = 0; v < 5; ++v) {
-        const double tmp = L.scan[s][v];
-        L.scan[s][v] = run[v];
-        run[v] += tmp;
-      }
+  for (int u = 0; u < kPer; ++u) {
+    const int k = tid * kPer + u;
+    xs[u] = 0.0;
+    qs[u] = 0.0;
+    if (k < cnt) {
+      const int j = lo + k;
+      const double img = tf_img(su, nv, P, j);
+      L.lu[k] = img;
+      xs[u] = (double)sx[tf_src(nv, j)] - xref;
+      qs[u] = img - r;
+      acc[0] += xs[u]; acc[1] += xs[u] * xs[u]; acc[2] += qs[u]; acc[3] += qs[u] * qs[u];
+      acc[4] += xs[u] * qs[u];
     }
+  }
+  // inclusive wave scan of the per-thread totals
+  double inc[5];
 #pragma unroll
-    for (int v = 0; v < 5; ++v) L.px[cnt][v] = run[v];
+  for (int v = 0; v < 5; ++v) inc[v] = acc[v];
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      const double y = __shfl_up(inc[v], o);
+      if (lane >= o) inc[v] += y;
+    }
+  }
+  if (lane == kWave - 1) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) L.scan[wv][v] = inc[v];
   }
   __syncthreads();
   double run[5];
 #pragma unroll
-  for (int v = 0; v < 5; ++v) run[v] = L.scan[tid][v];
-  for (int k = a; k < b; ++k) {
+  for (int v = 0; v < 5; ++v) {
+    double base = 0.0;
+    for (int w = 0; w < wv; ++w) base += L.scan[w][v];
+    run[v] = base + inc[v] - acc[v];  // exclusive prefix of this thread
+  }
 #pragma unroll
-    for (int v = 0; v < 5; ++v) L.px[k][v] = run[v];
-    const int j = lo + k;
-    const double xv = (double)sx[tf_src(nv, j)] - xref;
-    const double q = L.lu[k] - r;
-    run[0] += xv; run[1] += xv * xv; run[2] += q; run[3] += q * q; run[4] += xv * q;
+  for (int u = 0; u < kPer; ++u) {
+    const int k = tid * kPer + u;
+    if (k < cnt) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) L.px[k][v] = run[v];
+      run[0] += xs[u]; run[1] += xs[u] * xs[u]; run[2] += qs[u]; run[3] += qs[u] * qs[u];
+      run[4] += xs[u] * qs[u];
+      if (k == cnt - 1) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) L.px[cnt][v] = run[v];
+      }
+    }
+  }
+  if (cnt == 0 && tid == 0) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) L.px[0][v] = 0.0;
   }
   __syncthreads();
 }
@@ -385,25 +421,17 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
     TfBest best{-INFINITY, NAN, -1, -1, -1};
     const int nw = A.nw, nl = A.nl;
     const int nbnd = 3 + 4 * nl;
-    const int ext_lo = -nv, ext_hi = 2 * nv;  // extended index range searched
     int e0 = 0;
     const int te_max = max(1, kTfThreads / nw);
     int te = te_max;
     while (e0 < nE) {
       int e1 = min(nE, e0 + te);
-      // union range of windows of epochs [e0, e1)
+      // union range of the windows of epochs [e0, e1), from the bucket offsets
+      // (one bucket of slack each side; samples outside a window are excluded
+      // by the exact boundary searches)
       const double tlo = (double)e0 * A.dtep, thi = (double)(e1 - 1) * A.dtep;
-      int lo = 0, hi = 0;
-      if (tid == 0) {
-        lo = tf_search_global(su, nv, P, ext_lo, ext_hi, tlo, -hwm - marg, 0);
-        hi = tf_search_global(su, nv, P, lo, ext_hi, thi, hwm + marg, 1);
-        L.misc[2] = lo;
-        L.misc[3] = hi;
-      }
-      __syncthreads();
-      lo = L.misc[2];
-      hi = L.misc[3];
-      __syncthreads();
+      const int lo = tf_bucket_bound(cursor, nb, nv, P, bscale, tlo - hwm - marg, 0);
+      const int hi = tf_bucket_bound(cursor, nb, nv, P, bscale, thi + hwm + marg, 1);
       const int cnt = hi - lo;
       if (cnt > kTfChunk && e1 - e0 > 1) {  // shrink the tile
         te = max(1, (e1 - e0) / 2);
