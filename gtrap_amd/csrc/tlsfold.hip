@@ -40,6 +40,17 @@ __device__ __forceinline__ int ld_agent(const int *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// fmod(t, P) for t >= 0, P > 0, exactly: the remainder t - q*P of the
+// integer quotient q is representable, so one fused multiply-add returns it
+// exactly; q from the rounded division is corrected by one step when needed.
+__device__ __forceinline__ double tf_fmod(double t, double P) {
+  double q = floor(t / P);
+  double r = fma(-q, P, t);
+  if (r < 0.0) { q -= 1.0; r = fma(-q, P, t); }
+  else if (r >= P) { q += 1.0; r = fma(-q, P, t); }
+  return r;
+}
+
 struct TfBest {
   double sn, h;
   int w, l, e;
@@ -360,7 +371,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
     const double xref = (L.misc[0] < n) ? (double)xc[L.misc[0]] : 0.0;
     for (int i = tid; i < n; i += kTfThreads) {
       if (!(tc[i] >= 0.0f)) continue;
-      const double u = fmod((double)tc[i], P);
+      const double u = tf_fmod((double)tc[i], P);
       const int b = min(nb - 1, (int)(u * bscale));
       atomicAdd(&cursor[b], 1);
     }
@@ -389,7 +400,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
     // scatter (cursor[b] advances to the bucket end; starts recovered below)
     for (int i = tid; i < n; i += kTfThreads) {
       if (!(tc[i] >= 0.0f)) continue;
-      const double u = fmod((double)tc[i], P);
+      const double u = tf_fmod((double)tc[i], P);
       const int b = min(nb - 1, (int)(u * bscale));
       const int pos = atomicAdd(&cursor[b], 1);
       su[pos] = u;
