@@ -81,6 +81,6 @@ def test_gtls_dropin_module_kepler_shape(gpu):
 
 
 def test_gtls_rejects_non_power_of_two(gpu):
-    lc, tu = _image(nt=50, nq=1)
+    lc, tu = _image(nt=300, nq=1)
     with pytest.raises(ValueError):
         _run(lc, tu, 12, 4, 10, np.float32(0.2), np.float32(0.01), np.float32(0.0204), "normal")
