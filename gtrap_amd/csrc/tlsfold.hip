@@ -6,8 +6,9 @@
 //   u = fmod(t, P) (float64, exact), epoch e < ceil(P/dtep) at t0 = e*dtep,
 //   d = u - t0, or (u - P) - t0 if d >= P/2, or (u + P) - t0 if d < -P/2;
 //   L = m * (0.45*W/nl); regions A..E of tls.py:175-179 on d; mean over
-//   [-W/2, W/2]; best trial per period = first maximum in (W, L, epoch)
-//   order with NaN skipped (tlsfit, tls.py:217-231).
+//   [-W/2, W/2]; a trial needs n > 3 samples in [-W/2, W/2) (n - 3 degrees
+//   of freedom), else it is NaN; best trial per period = first maximum in
+//   (W, L, epoch) order with NaN skipped (tlsfit, tls.py:217-231).
 //
 // MI355X design.  One workgroup owns one (curve, period) work item at a time
 // (persistent grid, per-block global workspace):
@@ -167,7 +168,11 @@ __device__ __forceinline__ double tf_stat(const TfSums &c0, const TfSums &cAB,
   res += 2.0 * a * (ydD - q4 * yD) + a * a * (d2D - 2.0 * q4 * dD + q4 * q4 * nD);
   const double nn = nA + nB + nC + nD + nE;
   *Hout = H;
-  return -H / sqrt(res / (nn - 3.0));
+  // n - 3 degrees of freedom: fewer than 4 samples is no trial (the direct
+  // formula gives 0/0 there, the expanded residual only rounding noise)
+  if (nn <= 3.0) return __longlong_as_double(0x7ff8000000000000LL);
+  // the expanded residual of a perfect fit can round below zero
+  return -H / sqrt(fmax(res, 0.0) / (nn - 3.0));
 }
 
 struct TfArgs {
@@ -197,12 +202,14 @@ struct TfArgs {
 };
 
 // LDS layout for a staged range
+constexpr int kTfMaxTe = 48;  // epochs per LDS tile
+
+// Static LDS of a workgroup.  The per-(W, boundary) accumulators of the chunked
+// sweep and the per-(epoch, W) window table are sized at launch (dynamic LDS).
 struct TfLds {
-  double lu[kTfChunk];          // images (exact)
+  double lu[kTfChunk];          // staged images (exact)
   double px[kTfChunk + 1][5];   // exclusive prefix of x', x'^2, q, q^2, x'q
   double scan[kTfThreads / kWave][5];
-  int misc[8];
-  double wbnd[kTfMaxNw * kTfMaxBnd][6];  // chunk-mode accumulators (W x boundary)
   double red_sn[kTfThreads / kWave], red_h[kTfThreads / kWave];
   int red_w[kTfThreads / kWave], red_l[kTfThreads / kWave], red_e[kTfThreads / kWave];
 };
@@ -307,36 +314,39 @@ __device__ __forceinline__ TfSums tf_cum(const TfLds &L, int p, double s) {
 
 __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
   __shared__ TfLds L;
-  __shared__ int s_cnt;
+  __shared__ int s_cnt, s_i0;
+  __shared__ int64_t s_item;
+  extern __shared__ __align__(16) unsigned char tf_dyn[];
   const int tid = threadIdx.x;
-  unsigned char *ws = A.ws + (size_t)blockIdx.x * (size_t)A.ws_per_block;
+  const int nw = A.nw, nl = A.nl;
+  const int nbnd = 3 + 4 * nl;
+  double *wbnd = reinterpret_cast<double *>(tf_dyn);               // [nw*nbnd][6]
+  int *ewp = reinterpret_cast<int *>(wbnd + (size_t)nw * nbnd * 6);  // [kTfMaxTe*nw][3]
+  unsigned char *ws = A.ws + 256 + (size_t)blockIdx.x * (size_t)A.ws_per_block;
   double *su = reinterpret_cast<double *>(ws);
   float *sx = reinterpret_cast<float *>(su + A.nmax);
   int *cursor = reinterpret_cast<int *>(sx + A.nmax + (A.nmax & 1));
+  int *ticket = A.counter ? A.counter : reinterpret_cast<int *>(A.ws);
   double wmax = 0.0;
-  for (int w = 0; w < A.nw; ++w) wmax = fmax(wmax, A.widths[w]);
+  for (int w = 0; w < nw; ++w) wmax = fmax(wmax, A.widths[w]);
   const double hwm = 0.5 * wmax;
   const double marg = 1e-9;
-
-  __shared__ int64_t s_item;
   const int64_t nitems = A.nitems_dev ? (int64_t)*A.nitems_dev : A.nptot;
-  for (int64_t it = blockIdx.x;; it += gridDim.x) {
-    int64_t slot = it;
-    if (A.counter) {
-      if (tid == 0) s_item = (int64_t)atomicAdd(A.counter, 1);
-      __syncthreads();
-      slot = s_item;
-      __syncthreads();
-    }
+
+  for (;;) {
+    // ---- next work item (atomic ticket; packed batches run longest period first)
+    if (tid == 0) s_item = (int64_t)atomicAdd(ticket, 1);
+    __syncthreads();
+    const int64_t slot = s_item;
+    __syncthreads();
     if (slot >= nitems) break;
-    // slot -> (curve, period index)
     int c;
     int64_t item;
     if (A.item_curve) {
       c = A.item_curve[slot];
       item = A.item_pidx[slot];
     } else {
-      item = slot;
+      item = A.nptot - 1 - slot;
       int lo_c = 0, hi_c = A.ncurves - 1;
       while (lo_c < hi_c) {
         const int mid = (lo_c + hi_c + 1) >> 1;
@@ -351,64 +361,57 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
     const int nE = (int)ceil(P / A.dtep);
 
     // ---- 1. fold and bucket-sort by phase ------------------------------
-    if (tid == 0) s_cnt = 0;
+    if (tid == 0) { s_cnt = 0; s_i0 = n; }
     __syncthreads();
-    int myv = 0;
-    for (int i = tid; i < n; i += kTfThreads) myv += (tc[i] >= 0.0f) ? 1 : 0;
+    int myv = 0, myi0 = n;
+    for (int i = tid; i < n; i += kTfThreads)
+      if (tc[i] >= 0.0f) { ++myv; myi0 = min(myi0, i); }
     atomicAdd(&s_cnt, myv);
+    atomicMin(&s_i0, myi0);
     __syncthreads();
     const int nv = s_cnt;
+    // x_ref = flux of the first valid sample (any constant cancels in the statistic)
+    const double xref = (s_i0 < n) ? (double)xc[s_i0] : 0.0;
     const int nb = max(1, nv / 2);
     const double bscale = (double)nb / P;
     for (int b = tid; b <= nb; b += kTfThreads) cursor[b] = 0;
-    // xref: first valid sample's flux (any fixed reference cancels in the statistic)
-    if (tid == 0) {
-      int i0 = 0;
-      while (i0 < n && !(tc[i0] >= 0.0f)) ++i0;
-      L.misc[0] = i0;
-    }
     __syncthreads();
-    const double xref = (L.misc[0] < n) ? (double)xc[L.misc[0]] : 0.0;
     for (int i = tid; i < n; i += kTfThreads) {
       if (!(tc[i] >= 0.0f)) continue;
       const double u = tf_fmod((double)tc[i], P);
-      const int b = min(nb - 1, (int)(u * bscale));
-      atomicAdd(&cursor[b], 1);
+      atomicAdd(&cursor[min(nb - 1, (int)(u * bscale))], 1);
     }
     __syncthreads();
-    {  // exclusive scan of cursor[0..nb) in place, cursor[nb] = nv
+    {  // exclusive scan of cursor[0..nb) in place (per-thread runs + wave scan)
       const int per = (nb + kTfThreads - 1) / kTfThreads;
       const int a = tid * per, b = min(nb, a + per);
-      int s = 0;
-      for (int k = a; k < b; ++k) s += ld_agent(&cursor[k]);
-      L.misc[1] = 0;
-      __syncthreads();
-      // serial scan of per-thread sums via LDS
-      int *part = reinterpret_cast<int *>(L.scan);
-      part[tid] = s;
-      __syncthreads();
-      if (tid == 0) {
-        int run = 0;
-        for (int k = 0; k < kTfThreads; ++k) { const int v = part[k]; part[k] = run; run += v; }
+      int sum = 0;
+      for (int k = a; k < b; ++k) sum += ld_agent(&cursor[k]);
+      int inc = sum;
+      const int lane = tid & (kWave - 1), wv = tid / kWave;
+      for (int o = 1; o < kWave; o <<= 1) {
+        const int y = __shfl_up(inc, o);
+        if (lane >= o) inc += y;
       }
+      int *wsum = reinterpret_cast<int *>(L.scan);
+      if (lane == kWave - 1) wsum[wv] = inc;
       __syncthreads();
-      int run = part[tid];
+      int run = inc - sum;
+      for (int w = 0; w < wv; ++w) run += wsum[w];
       for (int k = a; k < b; ++k) { const int v = ld_agent(&cursor[k]); cursor[k] = run; run += v; }
       if (tid == 0) cursor[nb] = nv;
       __syncthreads();
     }
-    // scatter (cursor[b] advances to the bucket end; starts recovered below)
+    // scatter; afterwards cursor[b] is the end of bucket b
     for (int i = tid; i < n; i += kTfThreads) {
       if (!(tc[i] >= 0.0f)) continue;
       const double u = tf_fmod((double)tc[i], P);
-      const int b = min(nb - 1, (int)(u * bscale));
-      const int pos = atomicAdd(&cursor[b], 1);
+      const int pos = atomicAdd(&cursor[min(nb - 1, (int)(u * bscale))], 1);
       su[pos] = u;
       sx[pos] = xc[i];
     }
     __syncthreads();
-    // after the scatter cursor[b] == end of bucket b == start of bucket b+1;
-    // sort each bucket [start, end) by (u, x) with insertion sort
+    // sort each bucket by (u, x): insertion sort (buckets hold ~2 samples)
     for (int b = tid; b < nb; b += kTfThreads) {
       const int s0 = b == 0 ? 0 : ld_agent(&cursor[b - 1]);
       const int s1 = ld_agent(&cursor[b]);
@@ -425,57 +428,60 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
         sx[j + 1] = kx;
       }
     }
-    __threadfence_block();
     __syncthreads();
 
     // ---- 2/3. epoch tiles ---------------------------------------------
     TfBest best{-INFINITY, NAN, -1, -1, -1};
-    const int nw = A.nw, nl = A.nl;
-    const int nbnd = 3 + 4 * nl;
+    // first guess of the tile size from the mean phase density
+    const double per_epoch = (double)nv / P * A.dtep;
+    int te = (int)(((double)kTfChunk / fmax(per_epoch, 1e-9) - 2.0 * (hwm + marg) / A.dtep) - 3.0);
+    te = max(1, min(kTfMaxTe, te));
     int e0 = 0;
-    const int te_max = max(1, kTfThreads / nw);
-    int te = te_max;
     while (e0 < nE) {
-      int e1 = min(nE, e0 + te);
-      // union range of the windows of epochs [e0, e1), from the bucket offsets
-      // (one bucket of slack each side; samples outside a window are excluded
-      // by the exact boundary searches)
+      const int e1 = min(nE, e0 + te);
+      // union range of the windows of epochs [e0, e1) from the bucket offsets
+      // (one bucket of slack each side; the exact boundary searches exclude
+      // samples outside a window)
       const double tlo = (double)e0 * A.dtep, thi = (double)(e1 - 1) * A.dtep;
       const int lo = tf_bucket_bound(cursor, nb, nv, P, bscale, tlo - hwm - marg, 0);
       const int hi = tf_bucket_bound(cursor, nb, nv, P, bscale, thi + hwm + marg, 1);
       const int cnt = hi - lo;
-      if (cnt > kTfChunk && e1 - e0 > 1) {  // shrink the tile
+      if (cnt > kTfChunk && e1 - e0 > 1) {  // too dense here: halve the tile
         te = max(1, (e1 - e0) / 2);
         continue;
       }
       if (cnt <= kTfChunk) {
-        // ---- single stage: thread = (epoch, W) ----
         const double r = tlo;
         tf_stage(L, su, sx, nv, P, lo, cnt, r, xref);
-        const int slot = tid;
-        const int ep = e0 + slot / nw, w = slot % nw;
-        if (slot < (e1 - e0) * nw) {
-          const double t0 = (double)ep * A.dtep;
-          const double s = t0 - r;
-          const double W = A.widths[w];
+        const int nte = e1 - e0;
+        // (epoch, W) window table: p0 (d >= -W/2), pE (d >= W/2), pM (d > W/2)
+        for (int k = tid; k < nte * nw; k += kTfThreads) {
+          const int ep = e0 + k / nw, w = k % nw;
+          const double t0 = (double)ep * A.dtep, W = A.widths[w];
           const int p0 = tf_search(L.lu, 0, cnt, t0, -0.5 * W, 0);
           const int pE = tf_search(L.lu, p0, cnt, t0, 0.5 * W, 0);
           const int pM = tf_search(L.lu, pE, cnt, t0, 0.5 * W, 1);
-          const TfSums c0 = tf_cum(L, p0, s), cE = tf_cum(L, pE, s), cM = tf_cum(L, pM, s);
-          const double dl = 0.45 * W / (double)nl;
-          for (int m = 0; m < nl; ++m) {
-            double tAB, tBC, tCD, tDE;
-            tf_bounds(W, m, nl, tAB, tBC, tCD, tDE);
-            const int pAB = tf_search(L.lu, p0, pE, t0, tAB, 0);
-            const int pBC = tf_search(L.lu, pAB, pE, t0, tBC, 0);
-            const int pCD = tf_search(L.lu, pBC, pE, t0, tCD, 0);
-            const int pDE = tf_search(L.lu, pCD, pE, t0, tDE, 0);
-            double H;
-            const double sn = tf_stat(c0, tf_cum(L, pAB, s), tf_cum(L, pBC, s),
-                                      tf_cum(L, pCD, s), tf_cum(L, pDE, s), cE, cM, W,
-                                      (double)m * dl, &H);
-            if (tf_better(sn, w, m, ep, best)) best = TfBest{sn, H, w, m, ep};
-          }
+          ewp[3 * k] = p0; ewp[3 * k + 1] = pE; ewp[3 * k + 2] = pM;
+        }
+        __syncthreads();
+        // one thread per (epoch, W, L) trial, L fastest
+        for (int k = tid; k < nte * nw * nl; k += kTfThreads) {
+          const int m = k % nl, j = k / nl;
+          const int ep = e0 + j / nw, w = j % nw;
+          const double t0 = (double)ep * A.dtep, s = t0 - r, W = A.widths[w];
+          const int p0 = ewp[3 * j], pE = ewp[3 * j + 1], pM = ewp[3 * j + 2];
+          double tAB, tBC, tCD, tDE;
+          tf_bounds(W, m, nl, tAB, tBC, tCD, tDE);
+          const int pAB = tf_search(L.lu, p0, pE, t0, tAB, 0);
+          const int pBC = tf_search(L.lu, pAB, pE, t0, tBC, 0);
+          const int pCD = tf_search(L.lu, pBC, pE, t0, tCD, 0);
+          const int pDE = tf_search(L.lu, pCD, pE, t0, tDE, 0);
+          double H;
+          const double sn =
+              tf_stat(tf_cum(L, p0, s), tf_cum(L, pAB, s), tf_cum(L, pBC, s), tf_cum(L, pCD, s),
+                      tf_cum(L, pDE, s), tf_cum(L, pE, s), tf_cum(L, pM, s), W,
+                      (double)m * (0.45 * W / (double)nl), &H);
+          if (tf_better(sn, w, m, ep, best)) best = TfBest{sn, H, w, m, ep};
         }
         __syncthreads();
       } else {
@@ -483,7 +489,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
         const double t0 = (double)e0 * A.dtep;
         const int tot = nw * nbnd;
         for (int k = tid; k < tot; k += kTfThreads)
-          for (int v = 0; v < 6; ++v) L.wbnd[k][v] = 0.0;
+          for (int v = 0; v < 6; ++v) wbnd[6 * k + v] = 0.0;
         __syncthreads();
         for (int ca = lo; ca < hi; ca += kTfChunk) {
           const int cc = min(kTfChunk, hi - ca);
@@ -495,8 +501,9 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
             tf_bnd(A.widths[w], b, nl, cval, kind);
             const int p = tf_search(L.lu, 0, cc, t0, cval, kind);
             const TfSums cs = tf_cum(L, p, 0.0);
-            L.wbnd[k][0] += cs.n; L.wbnd[k][1] += cs.s1; L.wbnd[k][2] += cs.s2;
-            L.wbnd[k][3] += cs.d1; L.wbnd[k][4] += cs.d2; L.wbnd[k][5] += cs.xd;
+            double *acc = wbnd + 6 * k;
+            acc[0] += cs.n; acc[1] += cs.s1; acc[2] += cs.s2;
+            acc[3] += cs.d1; acc[4] += cs.d2; acc[5] += cs.xd;
           }
           __syncthreads();
         }
@@ -504,19 +511,19 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
           const int w = k / nl, m = k % nl;
           const double W = A.widths[w];
           auto get = [&](int b) {
-            const double *v = L.wbnd[w * nbnd + b];
+            const double *v = wbnd + 6 * (w * nbnd + b);
             return TfSums{v[0], v[1], v[2], v[3], v[4], v[5]};
           };
-          const double dl = 0.45 * W / (double)nl;
           double H;
           const double sn = tf_stat(get(0), get(3 + 4 * m), get(4 + 4 * m), get(5 + 4 * m),
-                                    get(6 + 4 * m), get(1), get(2), W, (double)m * dl, &H);
+                                    get(6 + 4 * m), get(1), get(2), W,
+                                    (double)m * (0.45 * W / (double)nl), &H);
           if (tf_better(sn, w, m, e0, best)) best = TfBest{sn, H, w, m, e0};
         }
         __syncthreads();
       }
       e0 = e1;
-      te = min(te_max, 2 * te);
+      te = min(kTfMaxTe, 2 * te);
     }
     // ---- reduce the per-thread bests (wave shuffles, then LDS) ----
     for (int o = kWave / 2; o > 0; o >>= 1) {
@@ -548,14 +555,24 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
   }
 }
 
+__global__ void tf_zero_ticket(int *p) { *p = 0; }
+
+static size_t tf_dyn_lds(int nw, int nl) {
+  return (size_t)nw * (3 + 4 * nl) * 6 * sizeof(double) + (size_t)kTfMaxTe * nw * 3 * sizeof(int);
+}
+
 }  // namespace gtrap
 
+// Workspace: a 256-byte header (work ticket) plus one slot per workgroup with
+// the phase-sorted curve (float64 phase, float32 flux) and the bucket offsets.
 extern "C" int64_t gtrap_tlsfold_ws_bytes(int32_t nmax, int32_t nblocks) {
   if (nmax < 1) nmax = 1;
   int64_t per = (int64_t)nmax * 8 + (int64_t)(nmax + 1) * 4 + (int64_t)(nmax / 2 + 2) * 4;
   per = (per + 255) / 256 * 256;  // keep every block's slot 256-byte aligned
-  return per * (int64_t)(nblocks > 0 ? nblocks : 1);
+  return 256 + per * (int64_t)(nblocks > 0 ? nblocks : 1);
 }
+
+static int64_t tf_slot_bytes(int32_t nmax) { return (gtrap_tlsfold_ws_bytes(nmax, 1) - 256); }
 
 extern "C" int gtrap_tlsfold(double *snr, double *hbest, int32_t *best_w, int32_t *best_l,
                              int32_t *best_e, const float *t, const float *x, const int64_t *off,
@@ -579,10 +596,12 @@ extern "C" int gtrap_tlsfold(double *snr, double *hbest, int32_t *best_w, int32_
   if (nptot == 0) return GTRAP_OK;
   gtrap::TfArgs A{t, x, off, len, periods, poff, nullptr, ncurves, nptot, widths, nw, nl, dtep,
                   snr, hbest, best_w, best_l, best_e, (unsigned char *)ws,
-                  gtrap_tlsfold_ws_bytes(nmax, 1), nmax, nullptr, nullptr, nullptr, nullptr};
+                  tf_slot_bytes(nmax), nmax, nullptr, nullptr, nullptr, nullptr};
   const int64_t g = nptot < nblocks ? nptot : nblocks;
-  hipLaunchKernelGGL(gtrap::tlsfold_kernel, dim3((unsigned)g), dim3(gtrap::kTfThreads), 0,
-                     gtrap::as_stream(stream), A);
+  hipStream_t st = gtrap::as_stream(stream);
+  hipLaunchKernelGGL(gtrap::tf_zero_ticket, dim3(1), dim3(1), 0, st, (int *)ws);
+  hipLaunchKernelGGL(gtrap::tlsfold_kernel, dim3((unsigned)g), dim3(gtrap::kTfThreads),
+                     gtrap::tf_dyn_lds(nw, nl), st, A);
   return gtrap::check_launch("tlsfold_kernel");
 }
 
@@ -606,10 +625,9 @@ extern "C" int gtrap_tlsfold_items(double *snr, double *hbest, int32_t *best_w, 
                 "gtrap_tlsfold_items: workspace too small");
   gtrap::TfArgs A{t, x, off, len, periods, nullptr, nullptr, 0, 0, widths, nw, nl, dtep,
                   snr, hbest, best_w, best_l, best_e, (unsigned char *)ws,
-                  gtrap_tlsfold_ws_bytes(nmax, 1), nmax, item_curve, item_pidx, nitems_dev,
-                  counter};
-  hipLaunchKernelGGL(gtrap::tlsfold_kernel, dim3((unsigned)nblocks), dim3(gtrap::kTfThreads), 0,
-                     gtrap::as_stream(stream), A);
+                  tf_slot_bytes(nmax), nmax, item_curve, item_pidx, nitems_dev, counter};
+  hipLaunchKernelGGL(gtrap::tlsfold_kernel, dim3((unsigned)nblocks), dim3(gtrap::kTfThreads),
+                     gtrap::tf_dyn_lds(nw, nl), gtrap::as_stream(stream), A);
   return gtrap::check_launch("tlsfold_kernel");
 }
 

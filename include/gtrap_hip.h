@@ -137,6 +137,21 @@ int gtrap_gtls(float *tlssn, float *tlsw, float *tlst0, float *tlsl, float *tlsh
                int32_t nl, int32_t nsc, float t0min, float dt, float wmax, float dw,
                float deltat, int mode, void *stream);
 
+/* ---- trapezoid least squares over an explicit grid (tlsfit) -------------- */
+
+/*
+ * Replaces the triple loop of gtrap/tls.py:82-110 (tlsfit) over
+ * get_chi2Hmax (tls.py:44-79): for every trial (w, l, k) (k fastest) with
+ * W = Wl[w], L = Ll[l], t0 = tl[k] on the float64 series (t, x) of n samples:
+ * H[trial] = Hmax, res[trial] = chi^2 / sigma^2, nout[trial] = samples in
+ * [-W/2, W/2), xsum[trial] = mean over [-W/2, W/2] (0 when divx == 0).  NaN
+ * samples are skipped in sums as numpy's nansum/nanmean.  xsum may be NULL.
+ */
+int gtrap_tlsgrid(const double *t, const double *x, int32_t n, const double *Wl, int32_t nW,
+                  const double *Ll, int32_t nL, const double *tl, int32_t nT, int divx,
+                  double sigma, double *H, double *res, int32_t *nout, double *xsum,
+                  void *stream);
+
 /* ---- periodic trapezoid least squares (tls_fold) --------------------------- */
 
 /*

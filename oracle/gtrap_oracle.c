@@ -169,7 +169,8 @@ void orc_geebls(float *blsp, uint32_t *phase1, uint32_t *phase2, const float *lc
 /* A [-W/2, -W/4-L/2), B [-W/4-L/2, -W/4+L/2), C [-W/4+L/2, W/4-L/2),        */
 /* D [W/4-L/2, W/4+L/2), E [W/4+L/2, W/2); mean over [-W/2, W/2] (:187);     */
 /* denA, numB, H = -numB/denA (:200-202); res (:204-206);                   */
-/* S/N = -H / sqrt(res / (n - 3)) (tls.py:100).                             */
+/* S/N = -H / sqrt(res / (n - 3)) (tls.py:100); a trial with n <= 3 (no      */
+/* degrees of freedom) is NaN.                                              */
 /* ------------------------------------------------------------------------ */
 static double fold_offset(double u, double t0, double P) {
   double d = u - t0;
@@ -227,6 +228,7 @@ double orc_tlsfold_trial(const float *t, const float *x, int n, double P, int e,
   const int nn = nA + nB + nC + nD + nE;
   if (Hout) *Hout = H;
   if (cnt) *cnt = nn;
+  if (nn <= 3) return NAN; /* n - 3 degrees of freedom: no trial below 4 samples */
   return -H / sqrt(res / (double)(nn - 3));
 }
 
@@ -272,6 +274,7 @@ static double trial_window(const double *d, const double *xv, int k, double W, i
   }
   const int nn = nA + nB + nC + nD + nE;
   if (Hout) *Hout = H;
+  if (nn <= 3) return NAN;
   return -H / sqrt(res / (double)(nn - 3));
 }
 

@@ -7,7 +7,8 @@ the sample's offset is d = u - t0, replaced by (u - P) - t0 when d >= P/2 and
 by (u + P) - t0 when d < -P/2.  For each width W and L = m*(0.45*W/nl)
 (m < nl; the reference's normal-mode grid, include/gtls_simple.h:69,167) the
 statistic of gtrap/tls.py:173-208 (get_chi2Hmax with divx=True, sigma=1) is
-evaluated on d, and S/N = -H/sqrt(res/(n-3)) (tls.py:229).  The best trial of
+evaluated on d, and S/N = -H/sqrt(res/(n-3)) (tls.py:229); a trial with
+n <= 3 samples in [-W/2, W/2) has no degrees of freedom and is NaN.  The best trial of
 a period is the first maximum in (W, L, epoch) order, NaN skipped
 (tls.py:217-231, nanargmax).
 
@@ -66,12 +67,11 @@ def fold_offsets(t, P, e, dtep):
     return d, ok
 
 
-def trial_numpy(t, x, P, e, W, m, nl, dtep, divx=True):
-    """tls.py's get_chi2Hmax masks and formulas (tls.py:173-208) on folded d."""
-    d, ok = fold_offsets(t, P, e, dtep)
-    xx = np.asarray(x)[ok].astype(np.float64)
-    L = m * (0.45 * W / nl)
-    tn = d
+def chi2hmax_numpy(tn, xx, W, L, divx=True, sigma=1.0):
+    """get_chi2Hmax of tls.py:173-208 on offsets tn (float64 numpy, nan-aware).
+    Returns (Hmax, res, n, xsum)."""
+    tn = np.asarray(tn, np.float64)
+    xx = np.asarray(xx, np.float64)
     maskA = (tn >= -W / 2) * (tn < -W / 4 - L / 2)
     maskB = (tn >= -W / 4 - L / 2) * (tn < -W / 4 + L / 2)
     maskC = (tn < W / 4 - L / 2) * (tn >= -W / 4 + L / 2)
@@ -97,5 +97,17 @@ def trial_numpy(t, x, P, e, W, m, nl, dtep, divx=True):
                + np.nansum((x_[maskE] + Hh) ** 2))
         res = res + np.nansum((x_[maskB] - H / L * (tn[maskB] + W / 4.0)) ** 2)
         res = res + np.nansum((x_[maskD] + H / L * (tn[maskD] - W / 4.0)) ** 2)
-        sn = -H / np.sqrt(res / (n - 3))
+        res = res / (sigma * sigma)
+    return H, res, n, xsum
+
+
+def trial_numpy(t, x, P, e, W, m, nl, dtep, divx=True):
+    """tls.py's get_chi2Hmax masks and formulas (tls.py:173-208) on folded d;
+    returns (S/N, H, n) with S/N = -H/sqrt(res/(n-3)) (tls.py:229)."""
+    d, ok = fold_offsets(t, P, e, dtep)
+    xx = np.asarray(x)[ok].astype(np.float64)
+    L = m * (0.45 * W / nl)
+    H, res, n, _ = chi2hmax_numpy(d, xx, W, L, divx=divx)
+    with np.errstate(all="ignore"):
+        sn = -H / np.sqrt(res / (n - 3)) if n > 3 else np.nan
     return sn, H, n

@@ -40,7 +40,7 @@ def _same(a, b):
 def test_served_equals_plain(gpu, use_graph):
     from gtrap_amd.serve import ServerConfig, TransitServer, search_plain
     reqs = _requests()
-    cfg = ServerConfig(items_per_step=500, max_curves=8, max_len=9000, hbm_budget=64 << 20,
+    cfg = ServerConfig(items_per_step=97, max_curves=8, max_len=9000, hbm_budget=64 << 20,
                        nblocks=256, use_graph=use_graph)
     plain = {r.rid: search_plain(r, cfg.widths, cfg.nl, cfg.dtep, cfg.r) for r in reqs}
     srv = TransitServer(cfg)
