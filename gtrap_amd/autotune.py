@@ -1,0 +1,132 @@
+"""Autotuner for execution parameters.
+
+Only parameters that do not change results are tuned: the number of
+persistent tls_fold workgroups, the serving step size (work items per step),
+and the detrend segment length.  Semantic parameters (bins, widths, L steps,
+grids, windows) are never touched, so every tuned configuration returns the
+same bits.
+
+Results are cached as JSON keyed by device name and workload signature in
+``$GTRAP_TUNE_CACHE`` (default ``~/.cache/gtrap_amd/tune.json``).
+"""
+from __future__ import annotations
+
+import json
+import os
+import time
+from pathlib import Path
+
+import numpy as np
+
+from . import _native
+
+
+def cache_path() -> Path:
+    p = os.environ.get("GTRAP_TUNE_CACHE")
+    return Path(p) if p else Path.home() / ".cache" / "gtrap_amd" / "tune.json"
+
+
+def _device_key() -> str:
+    import torch
+    return torch.cuda.get_device_name(0).replace(" ", "_")
+
+
+def load() -> dict:
+    try:
+        return json.loads(cache_path().read_text())
+    except (OSError, ValueError):
+        return {}
+
+
+def store(key: str, value) -> None:
+    d = load()
+    d[key] = value
+    p = cache_path()
+    p.parent.mkdir(parents=True, exist_ok=True)
+    p.write_text(json.dumps(d, indent=1, sort_keys=True))
+
+
+def lookup(key: str, default=None):
+    return load().get(f"{_device_key()}:{key}", default)
+
+
+def _time(fn, reps=3) -> float:
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    best = float("inf")
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    return best
+
+
+def tune_tlsfold_blocks(curves, periods_list, widths, nl=10, dtep=0.0204,
+                        candidates=(256, 512, 768, 1024, 1536, 2048), save=True) -> dict:
+    """Persistent workgroup count for tls_fold on a representative batch."""
+    from .tlsfold import Workspace, pack, tlsfold_device
+    batch = pack(curves, periods_list)
+    ws = Workspace()
+    times = {}
+    for nb in candidates:
+        times[nb] = _time(lambda: tlsfold_device(batch, widths, nl, dtep, nblocks=nb,
+                                                 workspace=ws))
+    best = min(times, key=times.get)
+    res = dict(nblocks=int(best), seconds={str(k): v for k, v in times.items()})
+    if save:
+        store(f"{_device_key()}:tlsfold.nblocks", int(best))
+    return res
+
+
+def tune_medfilt_seg(n, nq, r, candidates=(256, 512, 1024, 2048, 4096), save=True) -> dict:
+    import torch
+    from . import gfilter
+    x = torch.rand((n, nq), device="cuda") + 0.5
+    y = torch.empty_like(x)
+    times = {}
+    for seg in candidates:
+        if seg > n:
+            continue
+        times[seg] = _time(lambda: gfilter.medfilt_device(x, y, n, nq, r, "div", seg_len=seg))
+    best = min(times, key=times.get)
+    if save:
+        store(f"{_device_key()}:medfilt.seg.{r}.{nq >= 64}", int(best))
+    return dict(seg_len=int(best), seconds={str(k): v for k, v in times.items()})
+
+
+def tune_serve(trace_fn, candidates=((16384, 1024), (32768, 1024), (65536, 1024),
+                                     (32768, 2048), (65536, 2048)), save=True) -> dict:
+    """Serving step size and workgroups: served light curves/s on a trace
+    (``trace_fn()`` returns fresh requests, all available at once)."""
+    from .serve import ServerConfig, TransitServer
+    import torch
+    rates = {}
+    for ips, nb in candidates:
+        srv = TransitServer(ServerConfig(items_per_step=ips, nblocks=nb))
+        srv.run(trace_fn(), realtime=False)  # warm (graph capture)
+        reqs = trace_fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        srv.run(reqs, realtime=False)
+        torch.cuda.synchronize()
+        rates[f"{ips}x{nb}"] = len(reqs) / (time.perf_counter() - t0)
+        del srv
+    best = max(rates, key=rates.get)
+    ips, nb = (int(v) for v in best.split("x"))
+    if save:
+        store(f"{_device_key()}:serve.items_per_step", ips)
+        store(f"{_device_key()}:serve.nblocks", nb)
+    return dict(items_per_step=ips, nblocks=nb, rates=rates)
+
+
+if __name__ == "__main__":
+    import sys
+    from . import synth
+    _native.require_gpu()
+    reqs, _ = synth.config4_requests(24, seed=7)
+    print(tune_tlsfold_blocks([(r.t, r.flux) for r in reqs[:6]], [r.periods for r in reqs[:6]],
+                              np.linspace(0.04, 0.3, 10)), file=sys.stderr)
+    print(tune_medfilt_seg(65536, 256, 64), file=sys.stderr)
+    print(tune_serve(lambda: synth.config4_requests(48, seed=8)[0]), file=sys.stderr)

@@ -171,8 +171,9 @@ __device__ __forceinline__ double tf_stat(const TfSums &c0, const TfSums &cAB,
   // n - 3 degrees of freedom: fewer than 4 samples is no trial (the direct
   // formula gives 0/0 there, the expanded residual only rounding noise)
   if (nn <= 3.0) return __longlong_as_double(0x7ff8000000000000LL);
-  // the expanded residual of a perfect fit can round below zero
-  return -H / sqrt(fmax(res, 0.0) / (nn - 3.0));
+  // the expanded residual of a perfect fit can round below zero; a NaN
+  // residual (H = +-inf when denA == 0) must stay NaN, so no fmax here
+  return -H / sqrt((res < 0.0 ? 0.0 : res) / (nn - 3.0));
 }
 
 struct TfArgs {

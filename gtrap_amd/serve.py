@@ -180,6 +180,17 @@ class ServerConfig:
     seg_len: int = 512               # detrend rows per lane
     use_graph: bool = True
 
+    @classmethod
+    def tuned(cls, **kw) -> "ServerConfig":
+        """Defaults overridden by the autotuner's cached choices (if any)."""
+        from . import autotune
+        cfg = cls(**kw)
+        if "items_per_step" not in kw:
+            cfg.items_per_step = int(autotune.lookup("serve.items_per_step", cfg.items_per_step))
+        if "nblocks" not in kw:
+            cfg.nblocks = int(autotune.lookup("serve.nblocks", cfg.nblocks))
+        return cfg
+
 
 class TransitServer:
     def __init__(self, cfg: ServerConfig | None = None):
