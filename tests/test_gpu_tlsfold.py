@@ -67,15 +67,16 @@ def test_single_epoch_limit_and_nl_grid(gpu):
     _check([(t, x)], [np.array([2.5 * base, 4.0 * base])], widths, 4)
 
 
-def test_recovers_config1_like_signal(gpu):
+def test_config1_grid_matches_oracle(gpu):
+    """Config-1 curve (4096 samples, trapezoid at P = 3.7 d) on every 20th
+    period of the 1000-period grid plus the true period: the GPU's per-period
+    best trials equal the direct oracle's.  (tls.py's S/N is H over the
+    per-sample residual RMS, so the periodogram maximum need not sit at the
+    injected period; the comparison is with the oracle, not the injection.)"""
     from gtrap_amd import synth, tlsfold
     d = synth.config1()
-    per = d["periods"][::20]
-    per = np.sort(np.concatenate([per, [d["truth"]["P"]]]))
+    per = np.sort(np.concatenate([d["periods"][::20], [d["truth"]["P"]]]))
+    _check([(d["t"], d["flux"])], [per], d["widths"], 10)
     got = tlsfold.tlsfold([(d["t"], d["flux"])], [per], d["widths"], 10, 0.0204)[0]
-    best = int(np.nanargmax(got["sn"]))
-    assert abs(per[best] / d["truth"]["P"] - 1) < 1e-9
-    want = ot.tlsfold(d["t"], d["flux"], per[best:best + 1], d["widths"], 10, 0.0204)
-    np.testing.assert_allclose(got["sn"][best], want["sn"][0], rtol=1e-7)
-    assert (got["w"][best], got["l"][best], got["e"][best]) == \
-        (want["w"][0], want["l"][0], want["e"][0])
+    want = ot.tlsfold(d["t"], d["flux"], per, d["widths"], 10, 0.0204)
+    assert int(np.nanargmax(got["sn"])) == int(np.nanargmax(want["sn"]))
