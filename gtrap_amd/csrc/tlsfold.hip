@@ -29,17 +29,11 @@
 
 namespace gtrap {
 
-constexpr int kTfThreads = 256;
-constexpr int kTfChunk = 512;       // samples staged in LDS per tile/chunk
+constexpr int kTfThreads = 512;
+constexpr int kTfChunk = 2560;      // samples staged in LDS per tile/chunk (~123 KB)
 constexpr int kTfMaxNl = 12;
 constexpr int kTfMaxNw = 16;
 constexpr int kTfMaxBnd = 3 + 4 * kTfMaxNl;  // boundaries per W
-
-// Loads of words updated by global atomics go to L2 (agent scope), never to a
-// possibly stale L1 line.
-__device__ __forceinline__ int ld_agent(const int *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 // fmod(t, P) for t >= 0, P > 0, exactly: the remainder t - q*P of the
 // integer quotient q is representable, so one fused multiply-add returns it
@@ -50,6 +44,13 @@ __device__ __forceinline__ double tf_fmod(double t, double P) {
   if (r < 0.0) { q -= 1.0; r = fma(-q, P, t); }
   else if (r >= P) { q += 1.0; r = fma(-q, P, t); }
   return r;
+}
+
+// Total order on float bit patterns (signed-magnitude to two's complement);
+// breaks ties between samples of equal phase deterministically, NaN included.
+__device__ __forceinline__ int tf_xkey(float x) {
+  const int b = __float_as_int(x);
+  return b ^ ((b >> 31) & 0x7fffffff);
 }
 
 struct TfBest {
@@ -116,10 +117,10 @@ __device__ __forceinline__ int tf_bucket_bound(const int *cursor, int nb, int nv
   int idx;
   if (!upper) {
     const int b0 = bb - 1;
-    idx = b0 < 0 ? 0 : ld_agent(&cursor[b0]);  // start of bucket bb-1
+    idx = b0 < 0 ? 0 : cursor[b0];  // start of bucket bb-1
   } else {
     const int b1 = bb + 1 > nb - 1 ? nb - 1 : bb + 1;
-    idx = ld_agent(&cursor[b1]);  // end of bucket bb+1
+    idx = cursor[b1];  // end of bucket bb+1
   }
   idx += sh * nv;
   return idx < -nv ? -nv : (idx > 2 * nv ? 2 * nv : idx);
@@ -204,6 +205,10 @@ struct TfArgs {
 
 // LDS layout for a staged range
 constexpr int kTfMaxTe = 48;  // epochs per LDS tile
+constexpr int kTfMaxT = 4;    // chunked sweep: (epoch, W, boundary) sums per thread
+constexpr int kTfMaxBuckets = 30000;  // phase buckets (LDS counters, <= 120 KB)
+constexpr int kTfSortChunk = 7680;    // samples per in-LDS bucket-sort chunk (90 KB + ends)
+constexpr int kTfMaxBreaks = 64;      // sort chunks per period
 
 // Static LDS of a workgroup.  The per-(W, boundary) accumulators of the chunked
 // sweep and the per-(epoch, W) window table are sized at launch (dynamic LDS).
@@ -313,16 +318,19 @@ __device__ __forceinline__ TfSums tf_cum(const TfLds &L, int p, double s) {
   return o;
 }
 
+static_assert(sizeof(TfLds) >= (kTfMaxBuckets + 1) * sizeof(int), "bucket counters exceed LDS");
+static_assert(sizeof(TfLds) >= kTfSortChunk * (sizeof(double) + sizeof(float) + sizeof(int)),
+              "sort chunk exceeds LDS");
+
 __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
   __shared__ TfLds L;
-  __shared__ int s_cnt, s_i0;
+  __shared__ int s_cnt, s_i0, s_wsum[kTfThreads / kWave], s_brk[kTfMaxBreaks], s_nbrk;
   __shared__ int64_t s_item;
   extern __shared__ __align__(16) unsigned char tf_dyn[];
   const int tid = threadIdx.x;
   const int nw = A.nw, nl = A.nl;
   const int nbnd = 3 + 4 * nl;
-  double *wbnd = reinterpret_cast<double *>(tf_dyn);               // [nw*nbnd][6]
-  int *ewp = reinterpret_cast<int *>(wbnd + (size_t)nw * nbnd * 6);  // [kTfMaxTe*nw][3]
+  int *ewp = reinterpret_cast<int *>(tf_dyn);  // [kTfMaxTe*nw][3] window table
   unsigned char *ws = A.ws + 256 + (size_t)blockIdx.x * (size_t)A.ws_per_block;
   double *su = reinterpret_cast<double *>(ws);
   float *sx = reinterpret_cast<float *>(su + A.nmax);
@@ -362,74 +370,153 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
     const int nE = (int)ceil(P / A.dtep);
 
     // ---- 1. fold and bucket-sort by phase ------------------------------
+    // Counting sort into ~2-sample phase buckets with LDS counters (the tile
+    // LDS is idle during this phase), scatter to the workspace, then sort
+    // every bucket by (u, x) in LDS in bucket-aligned chunks.  The bucket ends
+    // are kept in the workspace for the tile phase.
+    // n/2 buckets (counted over all samples, so the counting pass also finds
+    // the number of valid samples and the first valid index)
+    const int nb = max(1, min(n / 2, kTfMaxBuckets));
+    const double bscale = (double)nb / P;
+    int *lcur = reinterpret_cast<int *>(&L);  // nb + 1 <= kTfMaxBuckets + 1 ints
     if (tid == 0) { s_cnt = 0; s_i0 = n; }
+    for (int b = tid; b <= nb; b += kTfThreads) lcur[b] = 0;
     __syncthreads();
-    int myv = 0, myi0 = n;
-    for (int i = tid; i < n; i += kTfThreads)
-      if (tc[i] >= 0.0f) { ++myv; myi0 = min(myi0, i); }
-    atomicAdd(&s_cnt, myv);
-    atomicMin(&s_i0, myi0);
+    {
+      int myv = 0, myi0 = n;
+      for (int i = tid; i < n; i += kTfThreads) {
+        if (!(tc[i] >= 0.0f)) continue;
+        ++myv;
+        myi0 = min(myi0, i);
+        const double u = tf_fmod((double)tc[i], P);
+        atomicAdd(&lcur[min(nb - 1, (int)(u * bscale))], 1);
+      }
+      atomicAdd(&s_cnt, myv);
+      atomicMin(&s_i0, myi0);
+    }
     __syncthreads();
     const int nv = s_cnt;
     // x_ref = flux of the first valid sample (any constant cancels in the statistic)
     const double xref = (s_i0 < n) ? (double)xc[s_i0] : 0.0;
-    const int nb = max(1, nv / 2);
-    const double bscale = (double)nb / P;
-    for (int b = tid; b <= nb; b += kTfThreads) cursor[b] = 0;
-    __syncthreads();
-    for (int i = tid; i < n; i += kTfThreads) {
-      if (!(tc[i] >= 0.0f)) continue;
-      const double u = tf_fmod((double)tc[i], P);
-      atomicAdd(&cursor[min(nb - 1, (int)(u * bscale))], 1);
-    }
-    __syncthreads();
-    {  // exclusive scan of cursor[0..nb) in place (per-thread runs + wave scan)
+    {  // exclusive scan of lcur[0..nb) in place (per-thread runs + wave scan)
       const int per = (nb + kTfThreads - 1) / kTfThreads;
-      const int a = tid * per, b = min(nb, a + per);
+      const int a0 = tid * per, b0 = min(nb, a0 + per);
       int sum = 0;
-      for (int k = a; k < b; ++k) sum += ld_agent(&cursor[k]);
+      for (int k = a0; k < b0; ++k) sum += lcur[k];
       int inc = sum;
       const int lane = tid & (kWave - 1), wv = tid / kWave;
       for (int o = 1; o < kWave; o <<= 1) {
         const int y = __shfl_up(inc, o);
         if (lane >= o) inc += y;
       }
-      int *wsum = reinterpret_cast<int *>(L.scan);
-      if (lane == kWave - 1) wsum[wv] = inc;
+      if (lane == kWave - 1) s_wsum[wv] = inc;
       __syncthreads();
       int run = inc - sum;
-      for (int w = 0; w < wv; ++w) run += wsum[w];
-      for (int k = a; k < b; ++k) { const int v = ld_agent(&cursor[k]); cursor[k] = run; run += v; }
-      if (tid == 0) cursor[nb] = nv;
+      for (int w = 0; w < wv; ++w) run += s_wsum[w];
+      for (int k = a0; k < b0; ++k) { const int v = lcur[k]; lcur[k] = run; run += v; }
+      if (tid == 0) lcur[nb] = nv;
       __syncthreads();
     }
-    // scatter; afterwards cursor[b] is the end of bucket b
+    // scatter; afterwards lcur[b] is the end of bucket b
     for (int i = tid; i < n; i += kTfThreads) {
       if (!(tc[i] >= 0.0f)) continue;
       const double u = tf_fmod((double)tc[i], P);
-      const int pos = atomicAdd(&cursor[min(nb - 1, (int)(u * bscale))], 1);
+      const int pos = atomicAdd(&lcur[min(nb - 1, (int)(u * bscale))], 1);
       su[pos] = u;
       sx[pos] = xc[i];
     }
     __syncthreads();
-    // sort each bucket by (u, x): insertion sort (buckets hold ~2 samples)
-    for (int b = tid; b < nb; b += kTfThreads) {
-      const int s0 = b == 0 ? 0 : ld_agent(&cursor[b - 1]);
-      const int s1 = ld_agent(&cursor[b]);
-      for (int k = s0 + 1; k < s1; ++k) {
-        const double ku = su[k];
-        const float kx = sx[k];
-        int j = k - 1;
-        while (j >= s0 && (su[j] > ku || (su[j] == ku && sx[j] > kx))) {
-          su[j + 1] = su[j];
-          sx[j + 1] = sx[j];
-          --j;
+    // bucket-aligned sort chunks of at most kTfSortChunk samples (greedy, from
+    // the LDS bucket ends), then the ends go to the workspace
+    if (tid == 0) {
+      int nbrk = 0, bb = 0;
+      s_brk[nbrk++] = 0;
+      while (bb < nb && nbrk < kTfMaxBreaks) {
+        const int s0 = bb == 0 ? 0 : lcur[bb - 1];
+        int lo_b = bb + 1, hi_b = nb;
+        while (lo_b < hi_b) {
+          const int mid = (lo_b + hi_b + 1) >> 1;
+          if (lcur[mid - 1] - s0 <= kTfSortChunk) lo_b = mid; else hi_b = mid - 1;
         }
-        su[j + 1] = ku;
-        sx[j + 1] = kx;
+        bb = lo_b;
+        s_brk[nbrk++] = bb;
       }
+      s_nbrk = (bb >= nb) ? nbrk : -1;  // -1: too many chunks, sort in place
     }
+    for (int b = tid; b <= nb; b += kTfThreads) cursor[b] = lcur[b];
     __syncthreads();
+    // sort every bucket by (u, x)
+    if (s_nbrk > 0) {
+      double *cu = reinterpret_cast<double *>(&L);
+      float *cx = reinterpret_cast<float *>(cu + kTfSortChunk);
+      int *cb = reinterpret_cast<int *>(cx + kTfSortChunk);  // bucket ends of the chunk
+      for (int c = 0; c + 1 < s_nbrk; ++c) {
+        const int bb = s_brk[c], be = s_brk[c + 1];
+        const int s0 = bb == 0 ? 0 : cursor[bb - 1];
+        const int s1 = cursor[be - 1];
+        const int cnt = s1 - s0;
+        if (cnt <= kTfSortChunk && be - bb <= kTfSortChunk) {
+          for (int k = tid; k < cnt; k += kTfThreads) { cu[k] = su[s0 + k]; cx[k] = sx[s0 + k]; }
+          for (int k = tid; k < be - bb; k += kTfThreads) cb[k] = cursor[bb + k] - s0;
+          __syncthreads();
+          // rank of every sample inside its bucket under the total order
+          // (u, key(x), position); equal (u, key(x)) pairs are the same bits,
+          // so their relative order never shows in the output
+          for (int k = tid; k < cnt; k += kTfThreads) {
+            const double ku = cu[k];
+            const float kx = cx[k];
+            const int kk = tf_xkey(kx);
+            const int b = min(be - bb - 1, max(0, min(nb - 1, (int)(ku * bscale)) - bb));
+            const int a1 = b == 0 ? 0 : cb[b - 1], a2 = cb[b];
+            int rank = 0;
+            for (int j = a1; j < a2; ++j) {
+              const double ju = cu[j];
+              const int jk = tf_xkey(cx[j]);
+              rank += (ju < ku || (ju == ku && (jk < kk || (jk == kk && j < k)))) ? 1 : 0;
+            }
+            const int dst = s0 + min(a1 + rank, cnt - 1);
+            su[dst] = ku;
+            sx[dst] = kx;
+          }
+          __syncthreads();
+        } else {
+          // one bucket larger than a chunk (strongly clustered phases)
+          for (int b = bb + tid; b < be; b += kTfThreads) {
+            const int a1 = b == 0 ? 0 : cursor[b - 1], a2 = cursor[b];
+            for (int k = a1 + 1; k < a2; ++k) {
+              const double ku = su[k];
+              const float kx = sx[k];
+              int j = k - 1;
+              while (j >= a1 && (su[j] > ku || (su[j] == ku && tf_xkey(sx[j]) > tf_xkey(kx)))) {
+                su[j + 1] = su[j];
+                sx[j + 1] = sx[j];
+                --j;
+              }
+              su[j + 1] = ku;
+              sx[j + 1] = kx;
+            }
+          }
+          __syncthreads();
+        }
+      }
+    } else {
+      for (int b = tid; b < nb; b += kTfThreads) {
+        const int a1 = b == 0 ? 0 : cursor[b - 1], a2 = cursor[b];
+        for (int k = a1 + 1; k < a2; ++k) {
+          const double ku = su[k];
+          const float kx = sx[k];
+          int j = k - 1;
+          while (j >= a1 && (su[j] > ku || (su[j] == ku && tf_xkey(sx[j]) > tf_xkey(kx)))) {
+            su[j + 1] = su[j];
+            sx[j + 1] = sx[j];
+            --j;
+          }
+          su[j + 1] = ku;
+          sx[j + 1] = kx;
+        }
+      }
+      __syncthreads();
+    }
 
     // ---- 2/3. epoch tiles ---------------------------------------------
     TfBest best{-INFINITY, NAN, -1, -1, -1};
@@ -486,42 +573,69 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
         }
         __syncthreads();
       } else {
-        // ---- chunked sweep of one epoch's window ----
-        const double t0 = (double)e0 * A.dtep;
-        const int tot = nw * nbnd;
-        for (int k = tid; k < tot; k += kTfThreads)
-          for (int v = 0; v < 6; ++v) wbnd[6 * k + v] = 0.0;
-        __syncthreads();
-        for (int ca = lo; ca < hi; ca += kTfChunk) {
-          const int cc = min(kTfChunk, hi - ca);
-          tf_stage(L, su, sx, nv, P, ca, cc, t0, xref);
-          for (int k = tid; k < tot; k += kTfThreads) {
-            const int w = k / nbnd, b = k % nbnd;
-            double cval;
-            int kind;
-            tf_bnd(A.widths[w], b, nl, cval, kind);
-            const int p = tf_search(L.lu, 0, cc, t0, cval, kind);
-            const TfSums cs = tf_cum(L, p, 0.0);
-            double *acc = wbnd + 6 * k;
-            acc[0] += cs.n; acc[1] += cs.s1; acc[2] += cs.s2;
-            acc[3] += cs.d1; acc[4] += cs.d2; acc[5] += cs.xd;
+        // ---- chunked sweep: a window larger than the LDS tile.  Up to
+        // kTfMaxT * kTfThreads (epoch, W, boundary) partial sums over a run
+        // of epochs are carried in registers across the chunks, then spilled
+        // into the (idle) prefix area for the trial pass.
+        const int tce = max(1, min(nE - e0, (kTfMaxT * kTfThreads) / (nw * nbnd)));
+        const double r = (double)e0 * A.dtep;
+        const int chi = tf_bucket_bound(cursor, nb, nv, P, bscale,
+                                        (double)(e0 + tce - 1) * A.dtep + hwm + marg, 1);
+        const int ntrip = tce * nw * nbnd;
+        double acc[kTfMaxT][6];
+#pragma unroll
+        for (int j = 0; j < kTfMaxT; ++j)
+#pragma unroll
+          for (int v = 0; v < 6; ++v) acc[j][v] = 0.0;
+        for (int ca = lo; ca < chi; ca += kTfChunk) {
+          const int cc = min(kTfChunk, chi - ca);
+          tf_stage(L, su, sx, nv, P, ca, cc, r, xref);
+#pragma unroll
+          for (int j = 0; j < kTfMaxT; ++j) {
+            const int k = tid + j * kTfThreads;
+            if (k < ntrip) {
+              const int el = k / (nw * nbnd), rem = k % (nw * nbnd);
+              const int w = rem / nbnd, b = rem % nbnd;
+              const double t0 = (double)(e0 + el) * A.dtep;
+              double cval;
+              int kind;
+              tf_bnd(A.widths[w], b, nl, cval, kind);
+              const int p = tf_search(L.lu, 0, cc, t0, cval, kind);
+              const TfSums cs = tf_cum(L, p, t0 - r);
+              acc[j][0] += cs.n; acc[j][1] += cs.s1; acc[j][2] += cs.s2;
+              acc[j][3] += cs.d1; acc[j][4] += cs.d2; acc[j][5] += cs.xd;
+            }
           }
           __syncthreads();
         }
-        for (int k = tid; k < nw * nl; k += kTfThreads) {
-          const int w = k / nl, m = k % nl;
+        double *accl = &L.px[0][0];  // tce*nw*nbnd*6 <= (kTfChunk+1)*5 doubles
+#pragma unroll
+        for (int j = 0; j < kTfMaxT; ++j) {
+          const int k = tid + j * kTfThreads;
+          if (k < ntrip)
+#pragma unroll
+            for (int v = 0; v < 6; ++v) accl[6 * k + v] = acc[j][v];
+        }
+        __syncthreads();
+        for (int k = tid; k < tce * nw * nl; k += kTfThreads) {
+          const int m = k % nl, jj = k / nl;
+          const int el = jj / nw, w = jj % nw;
           const double W = A.widths[w];
+          const double *base = accl + 6 * ((size_t)(el * nw + w) * nbnd);
           auto get = [&](int b) {
-            const double *v = wbnd + 6 * (w * nbnd + b);
+            const double *v = base + 6 * b;
             return TfSums{v[0], v[1], v[2], v[3], v[4], v[5]};
           };
           double H;
           const double sn = tf_stat(get(0), get(3 + 4 * m), get(4 + 4 * m), get(5 + 4 * m),
                                     get(6 + 4 * m), get(1), get(2), W,
                                     (double)m * (0.45 * W / (double)nl), &H);
-          if (tf_better(sn, w, m, e0, best)) best = TfBest{sn, H, w, m, e0};
+          if (tf_better(sn, w, m, e0 + el, best)) best = TfBest{sn, H, w, m, e0 + el};
         }
         __syncthreads();
+        e0 += tce;
+        te = 1;
+        continue;
       }
       e0 = e1;
       te = min(kTfMaxTe, 2 * te);
@@ -559,7 +673,8 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
 __global__ void tf_zero_ticket(int *p) { *p = 0; }
 
 static size_t tf_dyn_lds(int nw, int nl) {
-  return (size_t)nw * (3 + 4 * nl) * 6 * sizeof(double) + (size_t)kTfMaxTe * nw * 3 * sizeof(int);
+  (void)nl;
+  return (size_t)kTfMaxTe * nw * 3 * sizeof(int);
 }
 
 }  // namespace gtrap
