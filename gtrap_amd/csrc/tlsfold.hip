@@ -215,6 +215,11 @@ constexpr int kTfMaxBreaks = 64;      // sort chunks per period
 struct TfLds {
   double lu[kTfChunk];          // staged images (exact)
   double px[kTfChunk + 1][5];   // exclusive prefix of x', x'^2, q, q^2, x'q
+  // phase index of the staged range: tab[g] = first k with cell(lu[k]) >= g,
+  // cell(v) = clamp(floor((v - tbase) * tinv), 0, tnc - 1), tab[tnc] = cnt
+  int tab[kTfChunk + 1];
+  double tbase, tinv;
+  int tnc;
   double scan[kTfThreads / kWave][5];
   double red_sn[kTfThreads / kWave], red_h[kTfThreads / kWave];
   int red_w[kTfThreads / kWave], red_l[kTfThreads / kWave], red_e[kTfThreads / kWave];
@@ -237,6 +242,13 @@ __device__ __forceinline__ void tf_bnd(double W, int b, int nl, double &c, int &
 // exclusive prefix sums of (x', x'^2, q, q^2, x'q), q = image - r.  Thread t
 // owns kPer consecutive elements; the scan is a wave shuffle scan plus a
 // cross-wave fix-up.
+// cell of v in the phase index of a staged range
+__device__ __forceinline__ int tf_cell(double v, double base, double inv, int nc) {
+  double c = (v - base) * inv;
+  c = c < 0.0 ? 0.0 : (c > (double)(nc - 1) ? (double)(nc - 1) : c);
+  return (int)c;
+}
+
 constexpr int kPer = kTfChunk / kTfThreads;
 __device__ void tf_stage(TfLds &L, const double *su, const float *sx, int nv, double P, int lo,
                          int cnt, double r, double xref) {
@@ -275,6 +287,26 @@ __device__ void tf_stage(TfLds &L, const double *su, const float *sx, int nv, do
     for (int v = 0; v < 5; ++v) L.scan[wv][v] = inc[v];
   }
   __syncthreads();
+  {  // phase index: about one sample per cell; cells narrower than 1e-10 of
+     // the magnitudes (a range of near-equal phases) fall back to one cell
+    const double base = cnt > 0 ? L.lu[0] : 0.0, top = cnt > 0 ? L.lu[cnt - 1] : 0.0;
+    const int nc = cnt < 1 ? 1 : (cnt > kTfChunk ? kTfChunk : cnt);
+    const double span = top - base;
+    const double inv =
+        span > 1e-10 * (1.0 + fabs(base) + fabs(top)) * (double)nc ? (double)nc / span : 0.0;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int k = tid * kPer + u;
+      if (k < cnt) {
+        const int ck = tf_cell(L.lu[k], base, inv, nc);
+        const int cp = k == 0 ? -1 : tf_cell(L.lu[k - 1], base, inv, nc);
+        for (int g = cp + 1; g <= ck; ++g) L.tab[g] = k;
+      }
+    }
+    const int cl = cnt > 0 ? tf_cell(top, base, inv, nc) : -1;
+    for (int g = cl + 1 + tid; g <= nc; g += kTfThreads) L.tab[g] = cnt;
+    if (tid == 0) { L.tbase = base; L.tinv = inv; L.tnc = nc; }
+  }
   double run[5];
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
@@ -316,6 +348,22 @@ __device__ __forceinline__ TfSums tf_cum(const TfLds &L, int p, double s) {
   o.d2 = q2 - 2.0 * s * q1 + n * s * s;
   o.xd = xq - s * x1;
   return o;
+}
+
+// tf_search over the staged range through the phase index.  The answer over
+// the whole range lies in [tab[g-1], tab[g+2]] for g = cell(t0 + c): every
+// sample of a cell below g-1 is at least one cell width (far above the
+// rounding of the differences, see tf_stage) below t0 + c, every sample of a
+// cell above g+1 one cell width above it.  The result is clamped to [lo, hi],
+// which equals the search restricted to [lo, hi) (the predicate is monotone).
+__device__ __forceinline__ int tf_find(const TfLds &L, int lo, int hi, double t0, double c,
+                                       int kind) {
+  const int nc = L.tnc;
+  const int g = tf_cell(t0 + c, L.tbase, L.tinv, nc);
+  const int a = L.tab[g > 0 ? g - 1 : 0];
+  const int b = L.tab[g + 2 < nc ? g + 2 : nc];
+  const int r = tf_search(L.lu, a, b, t0, c, kind);
+  return r < lo ? lo : (r > hi ? hi : r);
 }
 
 static_assert(sizeof(TfLds) >= (kTfMaxBuckets + 1) * sizeof(int), "bucket counters exceed LDS");
@@ -546,9 +594,9 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
         for (int k = tid; k < nte * nw; k += kTfThreads) {
           const int ep = e0 + k / nw, w = k % nw;
           const double t0 = (double)ep * A.dtep, W = A.widths[w];
-          const int p0 = tf_search(L.lu, 0, cnt, t0, -0.5 * W, 0);
-          const int pE = tf_search(L.lu, p0, cnt, t0, 0.5 * W, 0);
-          const int pM = tf_search(L.lu, pE, cnt, t0, 0.5 * W, 1);
+          const int p0 = tf_find(L, 0, cnt, t0, -0.5 * W, 0);
+          const int pE = tf_find(L, p0, cnt, t0, 0.5 * W, 0);
+          const int pM = tf_find(L, pE, cnt, t0, 0.5 * W, 1);
           ewp[3 * k] = p0; ewp[3 * k + 1] = pE; ewp[3 * k + 2] = pM;
         }
         __syncthreads();
@@ -560,10 +608,10 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
           const int p0 = ewp[3 * j], pE = ewp[3 * j + 1], pM = ewp[3 * j + 2];
           double tAB, tBC, tCD, tDE;
           tf_bounds(W, m, nl, tAB, tBC, tCD, tDE);
-          const int pAB = tf_search(L.lu, p0, pE, t0, tAB, 0);
-          const int pBC = tf_search(L.lu, pAB, pE, t0, tBC, 0);
-          const int pCD = tf_search(L.lu, pBC, pE, t0, tCD, 0);
-          const int pDE = tf_search(L.lu, pCD, pE, t0, tDE, 0);
+          const int pAB = tf_find(L, p0, pE, t0, tAB, 0);
+          const int pBC = tf_find(L, pAB, pE, t0, tBC, 0);
+          const int pCD = tf_find(L, pBC, pE, t0, tCD, 0);
+          const int pDE = tf_find(L, pCD, pE, t0, tDE, 0);
           double H;
           const double sn =
               tf_stat(tf_cum(L, p0, s), tf_cum(L, pAB, s), tf_cum(L, pBC, s), tf_cum(L, pCD, s),
@@ -600,7 +648,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
               double cval;
               int kind;
               tf_bnd(A.widths[w], b, nl, cval, kind);
-              const int p = tf_search(L.lu, 0, cc, t0, cval, kind);
+              const int p = tf_find(L, 0, cc, t0, cval, kind);
               const TfSums cs = tf_cum(L, p, t0 - r);
               acc[j][0] += cs.n; acc[j][1] += cs.s1; acc[j][2] += cs.s2;
               acc[j][3] += cs.d1; acc[j][4] += cs.d2; acc[j][5] += cs.xd;
