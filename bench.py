@@ -1,4 +1,4 @@
-"""Benchmark: one JSON line on stdout (Measurement).
+"""Benchmark: one JSON line on stdout.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 
@@ -19,6 +19,11 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK = 78.6  # TFLOP/s, MI355X dense FP64
+# FP64 operations of one (epoch, W, L) trial from the tile prefix sums: 7
+# boundary sums shifted to epoch coordinates (10 each) and the closed-form
+# trapezoid fit, residual and S/N (122); +, -, *, /, sqrt count one each
+TRIAL_FLOPS = 192
 
 
 def log(msg: str) -> None:
@@ -188,6 +193,7 @@ def bench_serve(steps: int, warmup: int):
     traces = [synth.config4_requests(SERVE_REQUESTS, seed=1004 + k)[0]
               for k in range(warmup + steps)]
     srv = TransitServer(cfg)
+    srv.fold_log = []
     # warm-up (graph capture, allocator, code objects)
     for k in range(warmup):
         srv.run(traces[k], realtime=True)
@@ -223,21 +229,11 @@ def bench_serve(steps: int, warmup: int):
     plain = len(sub) / pwall
     ptrials = sum(synth.trial_count(r) for r in sub)
     log(f"plain: {len(sub)} requests in {pwall:.3f} s")
-    # dominant kernel: tlsfold over one full request, timed with events
-    big = max(sub, key=lambda r: len(r.t))
-    a = torch.cuda.Event(enable_timing=True)
-    b = torch.cuda.Event(enable_timing=True)
-    from gtrap_amd.tlsfold import pack, tlsfold_device
-    batch = pack([(big.t, big.flux)], [big.periods])
-    tlsfold_device(batch, cfg.widths, cfg.nl, cfg.dtep, nblocks=cfg.nblocks)
-    a.record()
-    tlsfold_device(batch, cfg.widths, cfg.nl, cfg.dtep, nblocks=cfg.nblocks)
-    b.record()
-    torch.cuda.synchronize()
-    kms = a.elapsed_time(b)
-    ktr = synth.trial_count(big)
-    flops = 60.0 * ktr  # closed-form statistic per (epoch, W, L) trial
-    ach = flops / (kms * 1e-3) / 1e12
+    # dominant kernel: the serving steps' tlsfold (item-list instantiation),
+    # every launch of the run (warm-up included, as a profiler sees them)
+    # bracketed by HIP events on the serving stream
+    nlaunch, kms, ktr, ksec = srv.fold_stats()
+    ach = TRIAL_FLOPS * ktr / ksec / 1e12
     # CPU baseline: the direct oracle on a bounded sample of periods
     from oracle import tlsfold as ot
     from oracle import medfilt as om
@@ -273,11 +269,11 @@ def bench_serve(steps: int, warmup: int):
                              "500-4000), 10 W x 10 L, detrend r=64 divide",
                     requests_per_step=SERVE_REQUESTS, items_per_step=cfg.items_per_step,
                     nblocks=cfg.nblocks, hbm_budget_gib=cfg.hbm_budget / 2 ** 30),
-        roofline=dict(bound="mfma", achieved=ach, peak=78.6, unit="TFLOP/s",
-                      frac=ach / 78.6, traffic=None, kernel="tlsfold_kernel",
-                      kernel_ms=kms, kernel_workload=f"1 curve n={len(big.t)}, "
-                                                     f"{len(big.periods)} periods",
-                      flops_per_trial=60),
+        roofline=dict(bound="mfma", achieved=ach, peak=FP64_PEAK, unit="TFLOP/s",
+                      frac=ach / FP64_PEAK, traffic=None,
+                      kernel="tlsfold_kernel<true> (serving step)", launches=nlaunch,
+                      kernel_ms=kms, trials_per_launch=ktr / max(nlaunch, 1),
+                      flops_per_trial=TRIAL_FLOPS),
         cpu_baseline=dict(value=cpu_lcs, unit="light curves/s", cores=1, kind="port",
                           trial_evals_per_s=cpu_tps,
                           sample=f"{done} of {len(r0.periods)} periods of one n={len(r0.t)} "
@@ -298,7 +294,7 @@ def main(argv=None):
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(the transit search shards by curve with no exchange; see DESIGN.md)",
+              "(the transit search shards by curve with no exchange between GPUs)",
               file=sys.stderr)
         return 2
     import torch

@@ -22,7 +22,8 @@ ARCH = os.environ.get("GTRAP_OFFLOAD_ARCH", "gfx950")
 
 # -ffp-contract=off: every fused multiply-add in the kernels is written out as
 # fmaf()/fma(), so results are exactly what the source says.
-# -packed-fp32-ops is disabled for all device code (Traps).
+# -packed-fp32-ops is disabled for all device code: some packed FP32 forms
+# have given wrong results on this hardware, without any error.
 HIPFLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     "-ffp-contract=off",

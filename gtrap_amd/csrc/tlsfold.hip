@@ -370,6 +370,10 @@ static_assert(sizeof(TfLds) >= (kTfMaxBuckets + 1) * sizeof(int), "bucket counte
 static_assert(sizeof(TfLds) >= kTfSortChunk * (sizeof(double) + sizeof(float) + sizeof(int)),
               "sort chunk exceeds LDS");
 
+// kItems: work items come from an item list (serving steps), else every
+// period of a packed batch, longest first.  Two instantiations, so profiles
+// tell the serving kernel from the per-request one.
+template <bool kItems>
 __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
   __shared__ TfLds L;
   __shared__ int s_cnt, s_i0, s_wsum[kTfThreads / kWave], s_brk[kTfMaxBreaks], s_nbrk;
@@ -399,7 +403,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
     if (slot >= nitems) break;
     int c;
     int64_t item;
-    if (A.item_curve) {
+    if (kItems) {
       c = A.item_curve[slot];
       item = A.item_pidx[slot];
     } else {
@@ -764,7 +768,7 @@ extern "C" int gtrap_tlsfold(double *snr, double *hbest, int32_t *best_w, int32_
   const int64_t g = nptot < nblocks ? nptot : nblocks;
   hipStream_t st = gtrap::as_stream(stream);
   hipLaunchKernelGGL(gtrap::tf_zero_ticket, dim3(1), dim3(1), 0, st, (int *)ws);
-  hipLaunchKernelGGL(gtrap::tlsfold_kernel, dim3((unsigned)g), dim3(gtrap::kTfThreads),
+  hipLaunchKernelGGL(gtrap::tlsfold_kernel<false>, dim3((unsigned)g), dim3(gtrap::kTfThreads),
                      gtrap::tf_dyn_lds(nw, nl), st, A);
   return gtrap::check_launch("tlsfold_kernel");
 }
@@ -790,9 +794,9 @@ extern "C" int gtrap_tlsfold_items(double *snr, double *hbest, int32_t *best_w, 
   gtrap::TfArgs A{t, x, off, len, periods, nullptr, nullptr, 0, 0, widths, nw, nl, dtep,
                   snr, hbest, best_w, best_l, best_e, (unsigned char *)ws,
                   tf_slot_bytes(nmax), nmax, item_curve, item_pidx, nitems_dev, counter};
-  hipLaunchKernelGGL(gtrap::tlsfold_kernel, dim3((unsigned)nblocks), dim3(gtrap::kTfThreads),
+  hipLaunchKernelGGL(gtrap::tlsfold_kernel<true>, dim3((unsigned)nblocks), dim3(gtrap::kTfThreads),
                      gtrap::tf_dyn_lds(nw, nl), gtrap::as_stream(stream), A);
-  return gtrap::check_launch("tlsfold_kernel");
+  return gtrap::check_launch("tlsfold_kernel<items>");
 }
 
 namespace gtrap {

@@ -20,9 +20,9 @@ Two paths compute exactly the same per-request result:
   longest first), then reduces the periodogram of every request whose last
   items ran in this step.  The step is a fixed-shape sequence of kernels whose
   variable parts (segment list, item list, counts) live in device descriptor
-  buffers, so it is captured once in a HIP graph and replayed.  Requests that
-  arrive while a step runs join the next one: their work overlaps with the
-  work of requests admitted earlier.
+  buffers, so it is captured once as HIP graphs (detrend, fold, statistics)
+  and replayed.  Requests that arrive while a step runs join the next one:
+  their work overlaps with the work of requests admitted earlier.
 
 Each work item's computation depends only on its own curve and period, so a
 result is the same alone and in any mix (bit for bit).
@@ -248,7 +248,8 @@ class TransitServer:
         self.slots: dict[int, _Slot] = {}
         self.order: list[int] = []   # resident slots, oldest first
         self.queue: list[Request] = []
-        self.graph = None
+        self.graphs = None
+        self.fold_log = None  # list of (start event, end event, trials) when set
         self.steps = 0
         self.stream = torch.cuda.current_stream()
 
@@ -279,41 +280,76 @@ class TransitServer:
         return new
 
     # -- one step ------------------------------------------------------------
-    def _record(self):
-        """The fixed kernel sequence of a step (captured once as a graph)."""
+    def _rec_pre(self):
+        """Ticket reset and detrend of the newly admitted curves."""
         cfg = self.cfg
         p = _native.ptr
-        st = _native.stream_handle()
         cnt = self.d_counts
         cnt[3:4].zero_()  # ticket counter (a fill kernel, graph-safe)
         _native.call("gtrap_medfilt_ragged", p(self.x), p(self.xd), p(self.d_off), p(self.d_len),
                      p(self.d_seg_curve), p(self.d_seg_start), p(cnt[0:1]), self.max_segs,
-                     cfg.r, 1 if cfg.mode == "div" else 0, cfg.seg_len, st)
+                     cfg.r, 1 if cfg.mode == "div" else 0, cfg.seg_len, _native.stream_handle())
+
+    def _rec_fold(self):
+        """The step's (curve, period) work items."""
+        cfg = self.cfg
+        p = _native.ptr
+        cnt = self.d_counts
         _native.call("gtrap_tlsfold_items", p(self.sn), p(self.h), p(self.bw), p(self.bl),
                      p(self.be), p(self.t), p(self.xd), p(self.d_off), p(self.d_len),
                      cfg.max_len, p(self.per), p(self.d_item_curve), p(self.d_item_pidx),
                      p(cnt[1:2]), p(cnt[3:4]), p(self.widths), int(self.widths.numel()), cfg.nl,
-                     float(cfg.dtep), p(self.ws_buf), int(self.ws_bytes), cfg.nblocks, st)
-        _native.call("gtrap_tlsstats", p(self.sn), p(self.d_poff), p(self.d_np),
-                     p(self.d_stat_curves), p(cnt[2:3]), cfg.max_curves, p(self.d_best),
-                     p(self.d_bsn), p(self.d_sde), st)
+                     float(cfg.dtep), p(self.ws_buf), int(self.ws_bytes), cfg.nblocks,
+                     _native.stream_handle())
 
-    def _launch(self):
+    def _rec_post(self):
+        """Periodogram statistics of the curves finished in this step."""
+        cfg = self.cfg
+        p = _native.ptr
+        _native.call("gtrap_tlsstats", p(self.sn), p(self.d_poff), p(self.d_np),
+                     p(self.d_stat_curves), p(self.d_counts[2:3]), cfg.max_curves,
+                     p(self.d_best), p(self.d_bsn), p(self.d_sde), _native.stream_handle())
+
+    def _launch(self, trials: int):
+        """One step's fixed kernel sequence: three graphs (detrend, fold,
+        statistics) captured on the first step and replayed after it.  With
+        ``fold_log`` set, HIP events on the serving stream bracket the fold."""
         import torch
-        if not self.cfg.use_graph:
-            self._record()
-            return
-        if self.graph is None:
-            # warm-up on a side stream as torch requires, then capture
+        parts = (self._rec_pre, self._rec_fold, self._rec_post)
+        if self.cfg.use_graph and self.graphs is None:
+            # the capture does not execute the kernels: this step runs eagerly
             torch.cuda.synchronize()
-            g = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(g):
-                self._record()
-            self.graph = g
-            # the capture does not execute the kernels: run this step eagerly
-            self._record()
-            return
-        self.graph.replay()
+            gs = []
+            for f in parts:
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g):
+                    f()
+                gs.append(g)
+            self.graphs = gs
+            run = parts
+        elif self.cfg.use_graph:
+            run = [g.replay for g in self.graphs]
+        else:
+            run = parts
+        run[0]()
+        if self.fold_log is not None:
+            ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+            ev[0].record(self.stream)
+            run[1]()
+            ev[1].record(self.stream)
+            self.fold_log.append((ev[0], ev[1], trials))
+        else:
+            run[1]()
+        run[2]()
+
+    def fold_stats(self):
+        """(launches, mean fold kernel ms, total trials, total fold seconds) of
+        the logged steps; synchronises."""
+        import torch
+        torch.cuda.synchronize()
+        ms = [a.elapsed_time(b) for a, b, _ in self.fold_log]
+        tr = sum(k for _, _, k in self.fold_log)
+        return len(ms), (float(np.mean(ms)) if ms else 0.0), tr, sum(ms) * 1e-3
 
     def step(self, now: float | None = None) -> list[Result]:
         """Admit arrived requests, run one device step, return finished results."""
@@ -347,6 +383,7 @@ class TransitServer:
         # work items: oldest requests first, up to items_per_step
         budget = cfg.items_per_step
         ic, ip, cost, done = [], [], [], []
+        nep = 0
         for s in self.order:
             if budget == 0:
                 break
@@ -360,6 +397,7 @@ class TransitServer:
             ic.append(np.full(take, s, np.int32))
             ip.append(np.arange(sl.poff + a, sl.poff + a + take, dtype=np.int64))
             cost.append(len(sl.req.t) + per * (100.0 / cfg.dtep))
+            nep += int(np.ceil(per / cfg.dtep).sum())
             sl.next_item += take
             budget -= take
             if sl.next_item == m:
@@ -387,7 +425,7 @@ class TransitServer:
             self.d_stat_curves[:len(done)].copy_(self.h_stat_curves[:len(done)],
                                                  non_blocking=True)
         self.d_counts[:3].copy_(self.h_counts[:3], non_blocking=True)
-        self._launch()
+        self._launch(nep * len(cfg.widths) * cfg.nl)
         self.steps += 1
         if not done:
             torch.cuda.current_stream().synchronize()
