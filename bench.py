@@ -24,6 +24,11 @@ FP64_PEAK = 78.6  # TFLOP/s, MI355X dense FP64
 # boundary sums shifted to epoch coordinates (10 each) and the closed-form
 # trapezoid fit, residual and S/N (122); +, -, *, /, sqrt count one each
 TRIAL_FLOPS = 192
+# HBM bytes per serving-step fold launch of the default serve workload,
+# measured offline: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch of
+# tlsfold_kernel<true>, scaled by the same counters over a 1 GiB copy
+# (factors 2.0 and 1.0)
+SERVE_FOLD_TRAFFIC = 7.565e10
 
 
 def log(msg: str) -> None:
@@ -270,7 +275,9 @@ def bench_serve(steps: int, warmup: int):
                     requests_per_step=SERVE_REQUESTS, items_per_step=cfg.items_per_step,
                     nblocks=cfg.nblocks, hbm_budget_gib=cfg.hbm_budget / 2 ** 30),
         roofline=dict(bound="mfma", achieved=ach, peak=FP64_PEAK, unit="TFLOP/s",
-                      frac=ach / FP64_PEAK, traffic=None,
+                      frac=ach / FP64_PEAK, traffic=SERVE_FOLD_TRAFFIC,
+                      traffic_source="rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE of this workload "
+                                     "(mean bytes per launch, copy-calibrated)",
                       kernel="tlsfold_kernel<true> (serving step)", launches=nlaunch,
                       kernel_ms=kms, trials_per_launch=ktr / max(nlaunch, 1),
                       flops_per_trial=TRIAL_FLOPS),
@@ -282,7 +289,44 @@ def bench_serve(steps: int, warmup: int):
     )
 
 
-WORKLOADS = {"gfilter": bench_gfilter, "geebls": bench_geebls, "serve": bench_serve}
+# ----------------------------------------------------------------------------
+# copy workload: a plain device copy of known size (HBM counter calibration)
+# ----------------------------------------------------------------------------
+COPY_BYTES = 1 << 30
+
+
+def bench_copy(steps: int, warmup: int):
+    """Copies a 1 GiB float32 buffer ``steps`` times: a profiler's
+    FETCH_SIZE / WRITE_SIZE per dispatch should read 1 GiB each."""
+    import torch
+    n = COPY_BYTES // 4
+    src = torch.rand(n, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(warmup):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    a.record()
+    for _ in range(steps):
+        dst.copy_(src)
+    b.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kms = a.elapsed_time(b) / steps
+    gbs = 2.0 * COPY_BYTES / (kms * 1e-3) / 1e9
+    return dict(metric="device copy bandwidth (read + write)", value=gbs, unit="GB/s",
+                ms_per_step=wall / steps * 1e3, dtype="f32",
+                config=dict(workload="copy of one 1 GiB float32 buffer", bytes=COPY_BYTES),
+                roofline=dict(bound="hbm", achieved=gbs, peak=HBM_PEAK_GBS, unit="GB/s",
+                              frac=gbs / HBM_PEAK_GBS, traffic=2.0 * COPY_BYTES,
+                              kernel="torch copy", kernel_ms=kms),
+                cpu_baseline=None)
+
+
+WORKLOADS = {"gfilter": bench_gfilter, "geebls": bench_geebls, "serve": bench_serve,
+             "copy": bench_copy}
 
 
 def main(argv=None):
