@@ -290,6 +290,44 @@ def bench_serve(steps: int, warmup: int):
 
 
 # ----------------------------------------------------------------------------
+# precision workload: BLS arithmetic variants against float64
+# ----------------------------------------------------------------------------
+def bench_precision(steps: int, warmup: int):
+    """256 curves x 262144 samples (2-minute cadence, 364 d), depth 1e-4 at
+    SNR 7-15, 2048 resolved frequencies around the transit period: SR and SDE
+    errors of each arithmetic variant against the float64 one."""
+    from gtrap_amd import precision, synth
+    nf = int(os.environ.get("GTRAP_BENCH_PREC_NF", "2048"))
+    log(f"precision: generating 256 curves x 262144 samples, {nf} frequencies")
+    d = synth.config5(nf=nf)
+    t0 = time.perf_counter()
+    res = precision.study(d)
+    wall = time.perf_counter() - t0
+    for k, v in res.items():
+        log(f"{k}: {v}")
+    own = res["f32/exact32/f32"]
+    g = d["grid"]
+    n, nq = d["lc"].shape
+    folds = float(nf) * nq * n
+    kms = own["ms"]
+    lds_bytes = nf * nq * (12.0 * n + 8.0 * (g["nb"] + g["kma"]))
+    ach = lds_bytes / (kms * 1e-3) / 1e12
+    lds_peak = 256 * 128 * 2.4e9 / 1e12
+    return dict(
+        metric="precision study: SR / SDE error vs float64 (geebls arithmetic variants)",
+        value=own["sde_rel_max"], unit="max relative SDE error (this build's arithmetic)",
+        ms_per_step=wall * 1e3, higher_is_better=False, dtype="f32",
+        variants=res,
+        config=dict(workload="precision: 256 curves x 262144 samples at 2 min, depth 1e-4, "
+                             f"SNR 7-15, {nf} frequencies, nb 1024, kma 23",
+                    n=n, nq=nq, nf=nf, nb=g["nb"]),
+        roofline=dict(bound="lds", achieved=ach, peak=lds_peak, unit="TB/s", frac=ach / lds_peak,
+                      traffic=None, kernel="blsprec_kernel f32/exact32/f32", kernel_ms=kms,
+                      sample_folds_per_s=folds / (kms * 1e-3)),
+        cpu_baseline=None)
+
+
+# ----------------------------------------------------------------------------
 # copy workload: a plain device copy of known size (HBM counter calibration)
 # ----------------------------------------------------------------------------
 COPY_BYTES = 1 << 30
@@ -326,7 +364,7 @@ def bench_copy(steps: int, warmup: int):
 
 
 WORKLOADS = {"gfilter": bench_gfilter, "geebls": bench_geebls, "serve": bench_serve,
-             "copy": bench_copy}
+             "copy": bench_copy, "precision": bench_precision}
 
 
 def main(argv=None):
@@ -348,7 +386,8 @@ def main(argv=None):
     res = WORKLOADS[args.workload](args.steps, args.warmup)
     line = dict(metric=res.pop("metric"), value=res.pop("value"), unit=res.pop("unit"),
                 n_gpus=1, steps=args.steps, warmup=args.warmup,
-                ms_per_step=res.pop("ms_per_step"), higher_is_better=True, scaling="weak",
+                ms_per_step=res.pop("ms_per_step"),
+                higher_is_better=res.pop("higher_is_better", True), scaling="weak",
                 vs_baseline=None, dtype=res.pop("dtype"), data="synthetic", **res)
     print(json.dumps(line), flush=True)
     return 0

@@ -25,6 +25,7 @@ SIGNATURES: dict[str, tuple] = {
                             _u32, _u32, _f32, _f32, _u32, _vp]),
     "gtrap_geebls_simple": (_int, [_vp, _vp, _vp, _u32, _u32, _u32, _u32, _u32, _u32, _u32,
                                    _f32, _f32, _u32, _vp]),
+    "gtrap_geebls_prec": (_int, [_int] + [_vp] * 10 + [_u32] * 7 + [_f64, _f64, _u32, _vp]),
     "gtrap_blsstats": (_int, [_vp, _vp, _vp, _u32, _u32, _u32, _f64, _f64, _vp, _vp, _vp, _vp,
                               _vp, _vp]),
     "gtrap_gtls": (_int, [_vp] * 7 + [_i32] * 5 + [_f32] * 5 + [_int, _vp]),

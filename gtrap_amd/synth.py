@@ -153,6 +153,39 @@ def config3(nq: int = 4096, n: int = 65536, seed: int = 1003, device=None):
     return f.astype(np.float32)
 
 
+def config5(nq: int = 256, n: int = 262144, seed: int = 1005, period: float = 7.3,
+            nf: int = 128):
+    """Precision workload: nq curves of n samples at a 2-minute cadence (364 d
+    for n = 262144), one periodic trapezoid per curve at P = ``period``
+    (random epoch, duty cycle q ~ U(11, 22)/1024, ingress T14/10, depth 1e-4),
+    white noise with sigma set per curve so that depth*sqrt(n q)/sigma is
+    U(7, 15).  The BLS grid is nf frequencies centred on 1/P with step
+    0.25 * (11/1024) / baseline (every transit resolved), nb = 1024,
+    kmi = 10, kma = 23, kkmi = n*kmi/nb, nthread 256."""
+    rng = np.random.default_rng(seed)
+    cad = 2.0 / 1440.0
+    t = np.arange(n) * cad
+    base = n * cad
+    lc = np.empty((n, nq), np.float32)
+    snr = rng.uniform(7, 15, nq)
+    qd = rng.uniform(11, 22, nq) / 1024
+    depth = 1e-4
+    for k in range(nq):
+        sigma = depth * np.sqrt(n * qd[k]) / snr[k]
+        t14 = qd[k] * period
+        f = 1.0 + rng.normal(0, sigma, n)
+        f -= trapezoid_dip(t, rng.uniform(0, period), period, depth, t14, t14 / 10)
+        lc[:, k] = f
+    nb, kmi, kma = 1024, 10, 23
+    df = 0.25 * (11 / 1024) / base
+    fmin = 1.0 / period - 0.5 * nf * df
+    return dict(t64=t, lc=lc, offset=lc.astype(np.float64).mean(0).astype(np.float32),
+                ntrue=np.full(nq, n, np.uint32),
+                grid=dict(nf=nf, nb=nb, kmi=kmi, kma=kma, kkmi=int(n * kmi / nb), fmin=fmin,
+                          df=df, nthread=256),
+                truth=dict(P=period, snr=snr, q=qd, depth=depth))
+
+
 def config4_requests(count: int = 200, seed: int = 1004, rate: float = 2000.0,
                      nlo: int = 2048, nhi: int = 70000, period_stride: int = 1):
     """Serving trace: Poisson arrivals (mean ``rate`` req/s), lengths

@@ -104,6 +104,29 @@ int gtrap_geebls_simple(float *blsp, const float *lc, const float *tu,
                         uint32_t kma, uint32_t kmi, uint32_t kkmi,
                         float fmin, float df, uint32_t nthread, void *stream);
 
+/*
+ * Arithmetic variants of the geebls kernel (include/geebls.h:1-167) for
+ * precision studies; the search itself is unchanged.  variant bits:
+ *   bit 0      flux read as binary16 lch[(n, nq)] = (lc - offset) * 2^k per
+ *              curve, hinv[q] = 2^-k (else float32 lc - offset[q]);
+ *   bits 1-2   accumulation: 0 float32 LDS atomics + float32 running sum
+ *              (the reference's arithmetic), 1 exact bin sums rounded to
+ *              float32 + float32 running sum (gtrap_geebls), 2 exact bins +
+ *              two-sum float32 running sum, 3 exact bins + float64 running sum
+ *              and statistic;
+ *   bit 3      fold in float64 from t64[(n, nq)]: f = fmin + df*jf,
+ *              ph = frac(t*f) (else the reference's float32 fold of tu).
+ * Valid samples are tu >= 0 in every variant.  sr[jf + nf*q] in float64;
+ * phase1/phase2 may be null.  Variant 2 (float32, exact, float32 fold) equals
+ * gtrap_geebls bit for bit.
+ */
+int gtrap_geebls_prec(int variant, double *sr, uint32_t *phase1, uint32_t *phase2,
+                      const float *lc, const uint16_t *lch, const float *hinv,
+                      const float *tu, const double *t64, const float *offset,
+                      const uint32_t *ntrue, uint32_t n, uint32_t nq, uint32_t nf,
+                      uint32_t nb, uint32_t kma, uint32_t kmi, uint32_t kkmi,
+                      double fmin, double df, uint32_t nthread, void *stream);
+
 /* ---- BLS statistics (getstat) -------------------------------------------- */
 
 /*
