@@ -29,6 +29,22 @@ def geebls_device(blsp, phase1, phase2, lc, tu, offset, ntrue, n, nq, nf, nb, km
                  float(fmin), float(df), int(nthread), _native.stream_handle(stream))
 
 
+def geebls_f64_device(sr, phase1, phase2, lc, tu, t64, offset, ntrue, n, nq, nf, nb, kma, kmi,
+                      kkmi, fmin, df, nthread=512, stream=None) -> None:
+    """Precision mode of geebls: the same search with a float64 fold
+    (f = fmin + df*jf, ph = frac(t*f) from float64 time ``t64``, interleaved
+    like ``tu``) and float64 bin sums, running sums and statistic; ``sr`` is
+    float64 at jf + nf*q.  Samples are valid where tu >= 0.  Over long
+    baselines the reference's float32 fold (include/geebls.h:68-70) moves
+    samples across bins; this mode does not."""
+    _native.require_gpu()
+    p = _native.ptr
+    _native.call("gtrap_geebls_prec", 14, p(sr), p(phase1), p(phase2), p(lc), None, None, p(tu),
+                 p(t64), p(offset), p(ntrue), int(n), int(nq), int(nf), int(nb), int(kma),
+                 int(kmi), int(kkmi), float(fmin), float(df), int(nthread),
+                 _native.stream_handle(stream))
+
+
 def _geebls_kernel(dev_blsp, dev_phase1, dev_phase2, dev_lc, dev_tu, dev_offset, dev_ntrue,
                    n, nf, nb, kma, kmi, kkmi, fmin, df, *, block, grid, shared, stream):
     if int(grid[0]) != int(nf):

@@ -66,3 +66,21 @@ def test_study_error_bounds(gpu):
     # of bins and running sums remains
     assert res["f32/exact32/f64"]["sr_rel_max"] < 1e-5
     assert res["f32/twosum32/f64"]["sr_rel_max"] < 1e-5
+
+
+def test_geebls_f64_mode_is_the_float64_variant(gpu):
+    import torch
+    from gtrap_amd import geebls, precision
+    d = _data(seed=14)
+    g = d["grid"]
+    inp = precision.Inputs(d)
+    want, w1, w2 = precision.run_variant(inp, *precision.TRUTH)
+    nq, nf = inp.nq, g["nf"]
+    sr = torch.empty(nq * nf, dtype=torch.float64, device="cuda")
+    p1 = torch.empty(nq * nf, dtype=torch.int32, device="cuda")
+    p2 = torch.empty(nq * nf, dtype=torch.int32, device="cuda")
+    geebls.geebls_f64_device(sr, p1, p2, inp.lc, inp.tu, inp.t64, inp.offset, inp.ntrue, inp.n,
+                             nq, nf, g["nb"], g["kma"], g["kmi"], g["kkmi"], g["fmin"], g["df"],
+                             nthread=g["nthread"])
+    np.testing.assert_array_equal(sr.cpu().numpy().reshape(nq, nf), want.cpu().numpy())
+    np.testing.assert_array_equal(p1.cpu().numpy().reshape(nq, nf), w1.cpu().numpy())
