@@ -225,16 +225,16 @@ struct TfLds {
   int red_w[kTfThreads / kWave], red_l[kTfThreads / kWave], red_e[kTfThreads / kWave];
 };
 
-// boundary b of width w (kind, offset): 0 -> -W/2 (>=), 1 -> W/2 (>=, region
-// end), 2 -> W/2 (>, mean end), 3 + 4m + {0,1,2,3} -> tAB, tBC, tCD, tDE (>=)
-__device__ __forceinline__ void tf_bnd(double W, int b, int nl, double &c, int &kind) {
+// boundary b of width W (kind, offset) from the width's row of the (W, L)
+// table: 0 -> -W/2 (>=), 1 -> W/2 (>=, region end), 2 -> W/2 (>, mean end),
+// 3 + 4m + {0,1,2,3} -> tAB, tBC, tCD, tDE of step m (>=)
+__device__ __forceinline__ void tf_bnd(double W, const double *wrow, int b, double &c,
+                                       int &kind) {
   if (b == 0) { c = -0.5 * W; kind = 0; return; }
   if (b == 1) { c = 0.5 * W; kind = 0; return; }
   if (b == 2) { c = 0.5 * W; kind = 1; return; }
   const int m = (b - 3) >> 2, r = (b - 3) & 3;
-  double tAB, tBC, tCD, tDE;
-  tf_bounds(W, m, nl, tAB, tBC, tCD, tDE);
-  c = r == 0 ? tAB : (r == 1 ? tBC : (r == 2 ? tCD : tDE));
+  c = wrow[5 * m + r];
   kind = 0;
 }
 
@@ -382,7 +382,19 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
   const int tid = threadIdx.x;
   const int nw = A.nw, nl = A.nl;
   const int nbnd = 3 + 4 * nl;
-  int *ewp = reinterpret_cast<int *>(tf_dyn);  // [kTfMaxTe*nw][3] window table
+  // dynamic LDS: per-(W, L) region offsets and L, the widths, then the
+  // per-(epoch, W) window table
+  double *wlt = reinterpret_cast<double *>(tf_dyn);  // [nw*nl][5]: tAB, tBC, tCD, tDE, L
+  double *sw = wlt + (size_t)5 * nw * nl;            // [nw]
+  int *ewp = reinterpret_cast<int *>(sw + nw);       // [kTfMaxTe*nw][3] window table
+  for (int k = tid; k < nw * nl; k += kTfThreads) {
+    const int w = k / nl, m = k % nl;
+    const double W = A.widths[w];
+    tf_bounds(W, m, nl, wlt[5 * k], wlt[5 * k + 1], wlt[5 * k + 2], wlt[5 * k + 3]);
+    wlt[5 * k + 4] = (double)m * (0.45 * W / (double)nl);
+  }
+  for (int w = tid; w < nw; w += kTfThreads) sw[w] = A.widths[w];
+  __syncthreads();
   unsigned char *ws = A.ws + 256 + (size_t)blockIdx.x * (size_t)A.ws_per_block;
   double *su = reinterpret_cast<double *>(ws);
   float *sx = reinterpret_cast<float *>(su + A.nmax);
@@ -597,7 +609,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
         // (epoch, W) window table: p0 (d >= -W/2), pE (d >= W/2), pM (d > W/2)
         for (int k = tid; k < nte * nw; k += kTfThreads) {
           const int ep = e0 + k / nw, w = k % nw;
-          const double t0 = (double)ep * A.dtep, W = A.widths[w];
+          const double t0 = (double)ep * A.dtep, W = sw[w];
           const int p0 = tf_find(L, 0, cnt, t0, -0.5 * W, 0);
           const int pE = tf_find(L, p0, cnt, t0, 0.5 * W, 0);
           const int pM = tf_find(L, pE, cnt, t0, 0.5 * W, 1);
@@ -608,10 +620,10 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
         for (int k = tid; k < nte * nw * nl; k += kTfThreads) {
           const int m = k % nl, j = k / nl;
           const int ep = e0 + j / nw, w = j % nw;
-          const double t0 = (double)ep * A.dtep, s = t0 - r, W = A.widths[w];
+          const double t0 = (double)ep * A.dtep, s = t0 - r, W = sw[w];
           const int p0 = ewp[3 * j], pE = ewp[3 * j + 1], pM = ewp[3 * j + 2];
-          double tAB, tBC, tCD, tDE;
-          tf_bounds(W, m, nl, tAB, tBC, tCD, tDE);
+          const double *bd = wlt + 5 * (w * nl + m);
+          const double tAB = bd[0], tBC = bd[1], tCD = bd[2], tDE = bd[3];
           const int pAB = tf_find(L, p0, pE, t0, tAB, 0);
           const int pBC = tf_find(L, pAB, pE, t0, tBC, 0);
           const int pCD = tf_find(L, pBC, pE, t0, tCD, 0);
@@ -620,7 +632,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
           const double sn =
               tf_stat(tf_cum(L, p0, s), tf_cum(L, pAB, s), tf_cum(L, pBC, s), tf_cum(L, pCD, s),
                       tf_cum(L, pDE, s), tf_cum(L, pE, s), tf_cum(L, pM, s), W,
-                      (double)m * (0.45 * W / (double)nl), &H);
+                      bd[4], &H);
           if (tf_better(sn, w, m, ep, best)) best = TfBest{sn, H, w, m, ep};
         }
         __syncthreads();
@@ -651,7 +663,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
               const double t0 = (double)(e0 + el) * A.dtep;
               double cval;
               int kind;
-              tf_bnd(A.widths[w], b, nl, cval, kind);
+              tf_bnd(sw[w], wlt + 5 * w * nl, b, cval, kind);
               const int p = tf_find(L, 0, cc, t0, cval, kind);
               const TfSums cs = tf_cum(L, p, t0 - r);
               acc[j][0] += cs.n; acc[j][1] += cs.s1; acc[j][2] += cs.s2;
@@ -672,7 +684,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
         for (int k = tid; k < tce * nw * nl; k += kTfThreads) {
           const int m = k % nl, jj = k / nl;
           const int el = jj / nw, w = jj % nw;
-          const double W = A.widths[w];
+          const double W = sw[w];
           const double *base = accl + 6 * ((size_t)(el * nw + w) * nbnd);
           auto get = [&](int b) {
             const double *v = base + 6 * b;
@@ -681,7 +693,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
           double H;
           const double sn = tf_stat(get(0), get(3 + 4 * m), get(4 + 4 * m), get(5 + 4 * m),
                                     get(6 + 4 * m), get(1), get(2), W,
-                                    (double)m * (0.45 * W / (double)nl), &H);
+                                    wlt[5 * (w * nl + m) + 4], &H);
           if (tf_better(sn, w, m, e0 + el, best)) best = TfBest{sn, H, w, m, e0 + el};
         }
         __syncthreads();
@@ -725,8 +737,8 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
 __global__ void tf_zero_ticket(int *p) { *p = 0; }
 
 static size_t tf_dyn_lds(int nw, int nl) {
-  (void)nl;
-  return (size_t)kTfMaxTe * nw * 3 * sizeof(int);
+  return (size_t)5 * nw * nl * sizeof(double) + (size_t)nw * sizeof(double) +
+         (size_t)kTfMaxTe * nw * 3 * sizeof(int);
 }
 
 }  // namespace gtrap
