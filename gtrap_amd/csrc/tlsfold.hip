@@ -133,41 +133,38 @@ struct TfSums {  // sums over a sample range in epoch coordinates
 // The statistic of one trial from cumulative sums at its boundaries
 // (cumulative from the window start): c0 = -W/2, cAB..cDE, cE = W/2 (<),
 // cM = W/2 (<=).  Returns S/N; writes H.
-__device__ __forceinline__ double tf_stat(const TfSums &c0, const TfSums &cAB,
-                                          const TfSums &cBC, const TfSums &cCD,
-                                          const TfSums &cDE, const TfSums &cE,
-                                          const TfSums &cM, double W, double L, double *Hout) {
-  // mean over [-W/2, W/2] (tls.py:187)
-  const double mcnt = cM.n - c0.n;
-  const double mean = (cM.s1 - c0.s1) / mcnt;
-  // region sums
-  const double nA = cAB.n - c0.n, nB = cBC.n - cAB.n, nC = cCD.n - cBC.n,
-               nD = cDE.n - cCD.n, nE = cE.n - cDE.n;
-  const double sA = cAB.s1 - c0.s1, sB = cBC.s1 - cAB.s1, sC = cCD.s1 - cBC.s1,
-               sD = cDE.s1 - cCD.s1, sE = cE.s1 - cDE.s1;
-  const double dB = cBC.d1 - cAB.d1, dD = cDE.d1 - cCD.d1;
-  const double d2B = cBC.d2 - cAB.d2, d2D = cDE.d2 - cCD.d2;
-  const double xdB = cBC.xd - cAB.xd, xdD = cDE.xd - cCD.xd;
-  // y = x' - mean
-  const double yA = sA - mean * nA, yB = sB - mean * nB, yC = sC - mean * nC,
-               yD = sD - mean * nD, yE = sE - mean * nE;
-  const double ydB = xdB - mean * dB, ydD = xdD - mean * dD;
+// Region sums of one trial: counts n*, flux sums s* (x' = x - x_ref), and for
+// the ingress/egress regions B and D the sums of d, d^2 and x'd (d = offset
+// from the epoch); the window's count, x' and x'^2 sums; the count and x' sum
+// of the closed window (for the mean).
+struct TfRegions {
+  double nA, nB, nC, nD, nE, sA, sB, sC, sD, sE;
+  double dB, dD, d2B, d2D, xdB, xdD;
+  double nAll, s1All, s2All, mcnt, sM;
+};
+
+// The statistic of one trial from its region sums (tls.py:44-79 expanded):
+// mean over [-W/2, W/2], H = -numB/denA, residual expanded in the sums,
+// S/N = -H / sqrt(res / (n - 3)).  Writes H.
+__device__ __forceinline__ double tf_eval(const TfRegions &g, double W, double L, double *Hout) {
+  const double mean = g.sM / g.mcnt;
+  const double yA = g.sA - mean * g.nA, yB = g.sB - mean * g.nB, yC = g.sC - mean * g.nC,
+               yD = g.sD - mean * g.nD, yE = g.sE - mean * g.nE;
+  const double ydB = g.xdB - mean * g.dB, ydD = g.xdD - mean * g.dD;
   const double xAEC = yA + yE + yC, xBD = yB + yD;
-  const double denA = 4.0 * L * L * (nA - nC + nE) + W * W * (nB + nD) + 8.0 * W * (dB - dD) +
-                      16.0 * (d2B + d2D);
+  const double denA = 4.0 * L * L * (g.nA - g.nC + g.nE) + W * W * (g.nB + g.nD) +
+                      8.0 * W * (g.dB - g.dD) + 16.0 * (g.d2B + g.d2D);
   const double numB = 8.0 * L * L * xAEC - 4.0 * W * L * xBD + 16.0 * L * (ydD - ydB);
   const double H = -numB / denA;
   const double h = H / 2.0;
   const double a = H / L;
   // sum of y^2 over A..E: samples in [-W/2, W/2)
-  const double nAll = cE.n - c0.n;
-  const double s1All = cE.s1 - c0.s1, s2All = cE.s2 - c0.s2;
-  const double y2 = s2All - 2.0 * mean * s1All + mean * mean * nAll;
+  const double y2 = g.s2All - 2.0 * mean * g.s1All + mean * mean * g.nAll;
   const double q4 = W / 4.0;
-  double res = y2 + 2.0 * h * (yA - yC + yE) + h * h * (nA + nC + nE);
-  res += -2.0 * a * (ydB + q4 * yB) + a * a * (d2B + 2.0 * q4 * dB + q4 * q4 * nB);
-  res += 2.0 * a * (ydD - q4 * yD) + a * a * (d2D - 2.0 * q4 * dD + q4 * q4 * nD);
-  const double nn = nA + nB + nC + nD + nE;
+  double res = y2 + 2.0 * h * (yA - yC + yE) + h * h * (g.nA + g.nC + g.nE);
+  res += -2.0 * a * (ydB + q4 * yB) + a * a * (g.d2B + 2.0 * q4 * g.dB + q4 * q4 * g.nB);
+  res += 2.0 * a * (ydD - q4 * yD) + a * a * (g.d2D - 2.0 * q4 * g.dD + q4 * q4 * g.nD);
+  const double nn = g.nA + g.nB + g.nC + g.nD + g.nE;
   *Hout = H;
   // n - 3 degrees of freedom: fewer than 4 samples is no trial (the direct
   // formula gives 0/0 there, the expanded residual only rounding noise)
@@ -175,6 +172,29 @@ __device__ __forceinline__ double tf_stat(const TfSums &c0, const TfSums &cAB,
   // the expanded residual of a perfect fit can round below zero; a NaN
   // residual (H = +-inf when denA == 0) must stay NaN, so no fmax here
   return -H / sqrt((res < 0.0 ? 0.0 : res) / (nn - 3.0));
+}
+
+// The statistic from cumulative sums at the trial's boundaries (cumulative
+// from the window start, epoch coordinates): c0 = -W/2, cAB..cDE, cE = W/2
+// (<), cM = W/2 (<=).
+__device__ __forceinline__ double tf_stat(const TfSums &c0, const TfSums &cAB,
+                                          const TfSums &cBC, const TfSums &cCD,
+                                          const TfSums &cDE, const TfSums &cE,
+                                          const TfSums &cM, double W, double L, double *Hout) {
+  TfRegions g;
+  g.mcnt = cM.n - c0.n;
+  g.sM = cM.s1 - c0.s1;
+  g.nA = cAB.n - c0.n; g.nB = cBC.n - cAB.n; g.nC = cCD.n - cBC.n;
+  g.nD = cDE.n - cCD.n; g.nE = cE.n - cDE.n;
+  g.sA = cAB.s1 - c0.s1; g.sB = cBC.s1 - cAB.s1; g.sC = cCD.s1 - cBC.s1;
+  g.sD = cDE.s1 - cCD.s1; g.sE = cE.s1 - cDE.s1;
+  g.dB = cBC.d1 - cAB.d1; g.dD = cDE.d1 - cCD.d1;
+  g.d2B = cBC.d2 - cAB.d2; g.d2D = cDE.d2 - cCD.d2;
+  g.xdB = cBC.xd - cAB.xd; g.xdD = cDE.xd - cCD.xd;
+  g.nAll = cE.n - c0.n;
+  g.s1All = cE.s1 - c0.s1;
+  g.s2All = cE.s2 - c0.s2;
+  return tf_eval(g, W, L, Hout);
 }
 
 struct TfArgs {
@@ -204,7 +224,9 @@ struct TfArgs {
 };
 
 // LDS layout for a staged range
-constexpr int kTfMaxTe = 48;  // epochs per LDS tile
+constexpr int kTfMaxTe = 48;     // epochs per window-table run
+constexpr int kTfMaxTile = 96;   // epochs per staged tile: with the reference at the
+                                 // tile centre, |q| stays within 48 epoch steps + W/2
 constexpr int kTfMaxT = 4;    // chunked sweep: (epoch, W, boundary) sums per thread
 constexpr int kTfMaxBuckets = 30000;  // phase buckets (LDS counters, <= 120 KB)
 constexpr int kTfSortChunk = 7680;    // samples per in-LDS bucket-sort chunk (90 KB + ends)
@@ -287,13 +309,13 @@ __device__ void tf_stage(TfLds &L, const double *su, const float *sx, int nv, do
     for (int v = 0; v < 5; ++v) L.scan[wv][v] = inc[v];
   }
   __syncthreads();
-  {  // phase index: about one sample per cell; cells narrower than 1e-10 of
+  {  // phase index: about one sample per cell; cells narrower than 1e-9 of
      // the magnitudes (a range of near-equal phases) fall back to one cell
     const double base = cnt > 0 ? L.lu[0] : 0.0, top = cnt > 0 ? L.lu[cnt - 1] : 0.0;
     const int nc = cnt < 1 ? 1 : (cnt > kTfChunk ? kTfChunk : cnt);
     const double span = top - base;
     const double inv =
-        span > 1e-10 * (1.0 + fabs(base) + fabs(top)) * (double)nc ? (double)nc / span : 0.0;
+        span > 1e-9 * (1.0 + fabs(base) + fabs(top)) * (double)nc ? (double)nc / span : 0.0;
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
       const int k = tid * kPer + u;
@@ -350,20 +372,86 @@ __device__ __forceinline__ TfSums tf_cum(const TfLds &L, int p, double s) {
   return o;
 }
 
-// tf_search over the staged range through the phase index.  The answer over
-// the whole range lies in [tab[g-1], tab[g+2]] for g = cell(t0 + c): every
-// sample of a cell below g-1 is at least one cell width (far above the
-// rounding of the differences, see tf_stage) below t0 + c, every sample of a
-// cell above g+1 one cell width above it.  The result is clamped to [lo, hi],
-// which equals the search restricted to [lo, hi) (the predicate is monotone).
+// tf_search over the staged range through the phase index, f = (t0 + c -
+// tbase) * tinv the target's fractional cell.  Cells are at least 1e-9 of the
+// magnitudes wide (tf_stage), so the rounding of f and of the offset test is
+// below 1e-4 of a cell: when f lies at least 1e-4 of a cell inside cell g,
+// every sample of a lower cell fails the test and every sample of a higher
+// cell passes it, and the answer lies in [tab[g], tab[g+1]].  Otherwise the
+// answer lies in [tab[g-1], tab[g+2]] (one whole cell of margin each side).
+// The result is clamped to [lo, hi], which equals the search restricted to
+// [lo, hi) (the test is monotone in the sorted images).
+constexpr double kTfEdge = 1e-4;
+__device__ __forceinline__ void tf_bracket(const TfLds &L, double t0, double c, int &a, int &b) {
+  const int nc = L.tnc;
+  const double f = (t0 + c - L.tbase) * L.tinv;
+  const double fl = floor(f), fr = f - fl;
+  if (fl >= 0.0 && fl <= (double)(nc - 1) && fr >= kTfEdge && fr <= 1.0 - kTfEdge) {
+    const int g = (int)fl;
+    a = L.tab[g];
+    b = L.tab[g + 1];
+  } else {
+    const int g = tf_cell(t0 + c, L.tbase, L.tinv, nc);
+    a = L.tab[g > 0 ? g - 1 : 0];
+    b = L.tab[g + 2 < nc ? g + 2 : nc];
+  }
+}
+
+__device__ __forceinline__ int tf_clamp(int r, int lo, int hi) {
+  return r < lo ? lo : (r > hi ? hi : r);
+}
+
 __device__ __forceinline__ int tf_find(const TfLds &L, int lo, int hi, double t0, double c,
                                        int kind) {
-  const int nc = L.tnc;
-  const int g = tf_cell(t0 + c, L.tbase, L.tinv, nc);
-  const int a = L.tab[g > 0 ? g - 1 : 0];
-  const int b = L.tab[g + 2 < nc ? g + 2 : nc];
-  const int r = tf_search(L.lu, a, b, t0, c, kind);
-  return r < lo ? lo : (r > hi ? hi : r);
+  int a, b;
+  tf_bracket(L, t0, c, a, b);
+  return tf_clamp(tf_search(L.lu, a, b, t0, c, kind), lo, hi);
+}
+
+// First index passing the offset test (kind 0, d >= c) in a bracket [a, b)
+// of at most kTfCount samples: a plus the number of failing samples, from
+// independent LDS reads (no loop, so several searches overlap).  Reads past
+// b stay inside lu and are masked.
+constexpr int kTfCount = 4;
+__device__ __forceinline__ int tf_count(const TfLds &L, int a, int b, double t0, double c) {
+  int r = a;
+#pragma unroll
+  for (int k = 0; k < kTfCount; ++k) {
+    const int idx = a + k < kTfChunk ? a + k : kTfChunk - 1;
+    const double d = L.lu[idx] - t0;
+    r += (a + k < b && !(d >= c)) ? 1 : 0;
+  }
+  return r;
+}
+
+// The statistic from the staged prefix sums at the trial's boundary indices:
+// raw differences first (x', x'^2 for the window and regions; q, q^2, x'q
+// only for B and D), shifted to epoch coordinates (d = q - s) per region.
+__device__ __forceinline__ double tf_stat_px(const TfLds &L, int p0, int pAB, int pBC, int pCD,
+                                             int pDE, int pE, int pM, double s, double W,
+                                             double Lw, double *Hout) {
+  const double x0 = L.px[p0][0], xAB = L.px[pAB][0], xBC = L.px[pBC][0], xCD = L.px[pCD][0],
+               xDE = L.px[pDE][0], xE = L.px[pE][0], xM = L.px[pM][0];
+  TfRegions g;
+  g.mcnt = (double)(pM - p0);
+  g.sM = xM - x0;
+  g.nA = (double)(pAB - p0); g.nB = (double)(pBC - pAB); g.nC = (double)(pCD - pBC);
+  g.nD = (double)(pDE - pCD); g.nE = (double)(pE - pDE);
+  g.sA = xAB - x0; g.sB = xBC - xAB; g.sC = xCD - xBC; g.sD = xDE - xCD; g.sE = xE - xDE;
+  const double q1B = L.px[pBC][2] - L.px[pAB][2], q2B = L.px[pBC][3] - L.px[pAB][3],
+               xqB = L.px[pBC][4] - L.px[pAB][4];
+  const double q1D = L.px[pDE][2] - L.px[pCD][2], q2D = L.px[pDE][3] - L.px[pCD][3],
+               xqD = L.px[pDE][4] - L.px[pCD][4];
+  g.dB = q1B - g.nB * s;
+  g.d2B = q2B - 2.0 * s * q1B + g.nB * s * s;
+  g.xdB = xqB - s * g.sB;
+  g.dD = q1D - g.nD * s;
+  g.d2D = q2D - 2.0 * s * q1D + g.nD * s * s;
+  g.xdD = xqD - s * g.sD;
+  g.nAll = (double)(pE - p0);
+  g.s1All = xE - x0;
+  g.s2All = L.px[pE][1] - L.px[p0][1];
+  return tf_eval(g, W, Lw, Hout);
 }
 
 static_assert(sizeof(TfLds) >= (kTfMaxBuckets + 1) * sizeof(int), "bucket counters exceed LDS");
@@ -395,6 +483,8 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
   }
   for (int w = tid; w < nw; w += kTfThreads) sw[w] = A.widths[w];
   __syncthreads();
+  // advance of the trial loop's (L, W, epoch) indices per kTfThreads trials
+  const int dm = kTfThreads % nl, dw = (kTfThreads / nl) % nw, dje = kTfThreads / (nl * nw);
   unsigned char *ws = A.ws + 256 + (size_t)blockIdx.x * (size_t)A.ws_per_block;
   double *su = reinterpret_cast<double *>(ws);
   float *sx = reinterpret_cast<float *>(su + A.nmax);
@@ -587,7 +677,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
     // first guess of the tile size from the mean phase density
     const double per_epoch = (double)nv / P * A.dtep;
     int te = (int)(((double)kTfChunk / fmax(per_epoch, 1e-9) - 2.0 * (hwm + marg) / A.dtep) - 3.0);
-    te = max(1, min(kTfMaxTe, te));
+    te = max(1, min(kTfMaxTile, te));
     int e0 = 0;
     while (e0 < nE) {
       const int e1 = min(nE, e0 + te);
@@ -603,39 +693,61 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
         continue;
       }
       if (cnt <= kTfChunk) {
-        const double r = tlo;
+        const double r = (double)((e0 + e1 - 1) / 2) * A.dtep;  // q reference: central epoch
         tf_stage(L, su, sx, nv, P, lo, cnt, r, xref);
-        const int nte = e1 - e0;
-        // (epoch, W) window table: p0 (d >= -W/2), pE (d >= W/2), pM (d > W/2)
-        for (int k = tid; k < nte * nw; k += kTfThreads) {
-          const int ep = e0 + k / nw, w = k % nw;
-          const double t0 = (double)ep * A.dtep, W = sw[w];
-          const int p0 = tf_find(L, 0, cnt, t0, -0.5 * W, 0);
-          const int pE = tf_find(L, p0, cnt, t0, 0.5 * W, 0);
-          const int pM = tf_find(L, pE, cnt, t0, 0.5 * W, 1);
-          ewp[3 * k] = p0; ewp[3 * k + 1] = pE; ewp[3 * k + 2] = pM;
+        // the staged tile is swept in runs of up to kTfMaxTe epochs
+        for (int eb = e0; eb < e1; eb += kTfMaxTe) {
+          const int nte = min(kTfMaxTe, e1 - eb);
+          // (epoch, W) window table: p0 (d >= -W/2), pE (d >= W/2), pM (d > W/2)
+          for (int k = tid; k < nte * nw; k += kTfThreads) {
+            const int ep = eb + k / nw, w = k % nw;
+            const double t0 = (double)ep * A.dtep, W = sw[w];
+            const int p0 = tf_find(L, 0, cnt, t0, -0.5 * W, 0);
+            const int pE = tf_find(L, p0, cnt, t0, 0.5 * W, 0);
+            const int pM = tf_find(L, pE, cnt, t0, 0.5 * W, 1);
+            ewp[3 * k] = p0; ewp[3 * k + 1] = pE; ewp[3 * k + 2] = pM;
+          }
+          __syncthreads();
+          // one thread per (epoch, W, L) trial, L fastest; the thread's
+          // (L, W, epoch) indices advance by kTfThreads trials per step
+          const int ntr = nte * nw * nl;
+          int m = tid % nl, w = (tid / nl) % nw, je = tid / (nl * nw);
+          for (int k = tid; k < ntr; k += kTfThreads) {
+            if (k != tid) {
+              m += dm;
+              const int c1 = m >= nl ? 1 : 0;
+              m -= c1 * nl;
+              w += dw + c1;
+              const int c2 = w >= nw ? 1 : 0;
+              w -= c2 * nw;
+              je += dje + c2;
+            }
+            const int j = je * nw + w, ep = eb + je;
+            const double t0 = (double)ep * A.dtep, s = t0 - r, W = sw[w];
+            const int p0 = ewp[3 * j], pE = ewp[3 * j + 1], pM = ewp[3 * j + 2];
+            const double *bd = wlt + 5 * (w * nl + m);
+            const double tAB = bd[0], tBC = bd[1], tCD = bd[2], tDE = bd[3];
+            int aAB, bAB, aBC, bBC, aCD, bCD, aDE, bDE;
+            tf_bracket(L, t0, tAB, aAB, bAB);
+            tf_bracket(L, t0, tBC, aBC, bBC);
+            tf_bracket(L, t0, tCD, aCD, bCD);
+            tf_bracket(L, t0, tDE, aDE, bDE);
+            int rAB = tf_count(L, aAB, bAB, t0, tAB), rBC = tf_count(L, aBC, bBC, t0, tBC),
+                rCD = tf_count(L, aCD, bCD, t0, tCD), rDE = tf_count(L, aDE, bDE, t0, tDE);
+            if (bAB - aAB > kTfCount) rAB = tf_search(L.lu, aAB, bAB, t0, tAB, 0);
+            if (bBC - aBC > kTfCount) rBC = tf_search(L.lu, aBC, bBC, t0, tBC, 0);
+            if (bCD - aCD > kTfCount) rCD = tf_search(L.lu, aCD, bCD, t0, tCD, 0);
+            if (bDE - aDE > kTfCount) rDE = tf_search(L.lu, aDE, bDE, t0, tDE, 0);
+            const int pAB = tf_clamp(rAB, p0, pE);
+            const int pBC = tf_clamp(rBC, pAB, pE);
+            const int pCD = tf_clamp(rCD, pBC, pE);
+            const int pDE = tf_clamp(rDE, pCD, pE);
+            double H;
+            const double sn = tf_stat_px(L, p0, pAB, pBC, pCD, pDE, pE, pM, s, W, bd[4], &H);
+            if (tf_better(sn, w, m, ep, best)) best = TfBest{sn, H, w, m, ep};
+          }
+          __syncthreads();
         }
-        __syncthreads();
-        // one thread per (epoch, W, L) trial, L fastest
-        for (int k = tid; k < nte * nw * nl; k += kTfThreads) {
-          const int m = k % nl, j = k / nl;
-          const int ep = e0 + j / nw, w = j % nw;
-          const double t0 = (double)ep * A.dtep, s = t0 - r, W = sw[w];
-          const int p0 = ewp[3 * j], pE = ewp[3 * j + 1], pM = ewp[3 * j + 2];
-          const double *bd = wlt + 5 * (w * nl + m);
-          const double tAB = bd[0], tBC = bd[1], tCD = bd[2], tDE = bd[3];
-          const int pAB = tf_find(L, p0, pE, t0, tAB, 0);
-          const int pBC = tf_find(L, pAB, pE, t0, tBC, 0);
-          const int pCD = tf_find(L, pBC, pE, t0, tCD, 0);
-          const int pDE = tf_find(L, pCD, pE, t0, tDE, 0);
-          double H;
-          const double sn =
-              tf_stat(tf_cum(L, p0, s), tf_cum(L, pAB, s), tf_cum(L, pBC, s), tf_cum(L, pCD, s),
-                      tf_cum(L, pDE, s), tf_cum(L, pE, s), tf_cum(L, pM, s), W,
-                      bd[4], &H);
-          if (tf_better(sn, w, m, ep, best)) best = TfBest{sn, H, w, m, ep};
-        }
-        __syncthreads();
       } else {
         // ---- chunked sweep: a window larger than the LDS tile.  Up to
         // kTfMaxT * kTfThreads (epoch, W, boundary) partial sums over a run
@@ -702,7 +814,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
         continue;
       }
       e0 = e1;
-      te = min(kTfMaxTe, 2 * te);
+      te = min(kTfMaxTile, 2 * te);
     }
     // ---- reduce the per-thread bests (wave shuffles, then LDS) ----
     for (int o = kWave / 2; o > 0; o >>= 1) {
