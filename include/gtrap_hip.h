@@ -127,6 +127,50 @@ int gtrap_geebls_prec(int variant, double *sr, uint32_t *phase1, uint32_t *phase
                       uint32_t nb, uint32_t kma, uint32_t kmi, uint32_t kkmi,
                       double fmin, double df, uint32_t nthread, void *stream);
 
+/* ---- image assembly ---------------------------------------------------- */
+
+/*
+ * Replaces the array assembly of gtrap/read_keplerlc.py:23-57 (load_keplc)
+ * and :123-146 (throw_kepintarray) for curves already in device memory:
+ * curve q's samples are t/flux/cno[off[q] + i], i < len[q] (maxlen >= every
+ * len[q]).  Output: interleaved (n, nq) float32 lc and tu; slot
+ * j = cno[i] - cno[off[q]] gets (float)flux and (float)(t - t0) unless flux is
+ * NaN (t0 = t[off[q]] if offt, else 0); other slots hold fillv / inval; gap
+ * slots (tu < 0) then get the curve's minimum flux slot at even j and its
+ * maximum at odd j.  ntrue[q] = slots with tu >= 0, hfill/lfill = those
+ * extrema, t0[q].  Samples whose slot is outside [0, n) are skipped and
+ * counted in the device int *nbad (the reference stops for j >= n).
+ */
+int gtrap_image_build(float *lc, float *tu, uint32_t *ntrue, float *hfill, float *lfill,
+                      double *t0, const double *t, const double *flux, const int64_t *cno,
+                      const int64_t *off, const int32_t *len, int32_t nq, int32_t n,
+                      int32_t maxlen, float fillv, float inval, int offt, int32_t *nbad,
+                      void *stream);
+
+/* ---- synthetic injection ------------------------------------------------- */
+
+/*
+ * Replaces the gammagen kernel of gtrap/ggenmock.py:18-79: x[id] ~ Gamma(a,
+ * rate b) for id < count (Marsaglia-Tsang, u^(1/a) boost for a <= 1, at most
+ * 10000 proposals; a slot whose proposals all fail is NaN and counted in the
+ * device int *nfail).  Random words: Philox4x32-10 keyed by seed, counter
+ * (id, proposal), so x depends only on (seed, id).
+ */
+int gtrap_gamma(float *x, int64_t count, double a, double b, uint64_t seed, int32_t *nfail,
+                void *stream);
+
+/*
+ * Mock transit injection (the role of gtrap/genmock.py:9-43 for trapezoids):
+ * on the interleaved (n, nq) image, subtract (mode 0) or multiply by one
+ * minus (mode 1) the dip of the gtrap/tls.py:2-15 trapezoid, per curve q:
+ * depth[q] on |d| < W/4 - L/2, linear to 0 at |d| = W/4 + L/2 (W = width[q],
+ * L = ell[q]), d the offset of tu from the nearest t0[q] + k period[q]
+ * (period <= 0: one event).  Samples with tu < 0 are left unchanged.
+ */
+int gtrap_inject_trapezoid(float *lc, const float *tu, int32_t n, int32_t nq, const double *t0,
+                           const double *period, const double *depth, const double *width,
+                           const double *ell, int mode, void *stream);
+
 /* ---- BLS statistics (getstat) -------------------------------------------- */
 
 /*
