@@ -28,7 +28,7 @@ TRIAL_FLOPS = 192
 # measured offline: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch of
 # tlsfold_kernel<true>, scaled by the same counters over a 1 GiB copy
 # (factors 2.0 and 1.0)
-SERVE_FOLD_TRAFFIC = 7.565e10
+SERVE_FOLD_TRAFFIC = 7.39e10
 
 
 def log(msg: str) -> None:
