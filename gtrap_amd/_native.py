@@ -29,6 +29,8 @@ SIGNATURES: dict[str, tuple] = {
     "gtrap_image_build": (_int, [_vp] * 11 + [_i32, _i32, _i32, _f32, _f32, _int, _vp, _vp]),
     "gtrap_gamma": (_int, [_vp, _i64, _f64, _f64, C.c_uint64, _vp, _vp]),
     "gtrap_inject_trapezoid": (_int, [_vp, _vp, _i32, _i32] + [_vp] * 5 + [_int, _vp]),
+    "gtrap_peakstats_ws_bytes": (_i64, [_i32, _i32]),
+    "gtrap_peakstats": (_int, [_vp] * 5 + [_i32] * 4 + [_vp, _i64, _i32, _vp]),
     "gtrap_blsstats": (_int, [_vp, _vp, _vp, _u32, _u32, _u32, _f64, _f64, _vp, _vp, _vp, _vp,
                               _vp, _vp]),
     "gtrap_gtls": (_int, [_vp] * 7 + [_i32] * 5 + [_f32] * 5 + [_int, _vp]),

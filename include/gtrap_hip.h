@@ -127,6 +127,24 @@ int gtrap_geebls_prec(int variant, double *sr, uint32_t *phase1, uint32_t *phase
                       uint32_t nb, uint32_t kma, uint32_t kmi, uint32_t kkmi,
                       double fmin, double df, uint32_t nthread, void *stream);
 
+/* ---- post-search peak statistics ----------------------------------------- */
+
+/*
+ * Replaces the statistics block after the search in gtrap/gtls.py:244-275 and
+ * kepler/gtls_mockkepler.py:336-376, with the peak finder
+ * gtrap/detect_peaks.py:10-138 (edge 'rising', minimum distance mpd), for
+ * each series q of the interleaved sn[k*nq + q], k < nt (t0 alike, may be
+ * null).  Per q: st[8q..8q+7] = maxsn, median, std (population, over
+ * sn > 0), far, diff, pssn, pinterval, 0; cnt[2q] = peaks, cnt[2q+1] =
+ * count(sn > 0); top[ntop*q + t] = the t-th highest peak's index (-1 past
+ * the last).  Ties in height: the larger index first.  Workspace:
+ * gtrap_peakstats_ws_bytes(nt, nblocks) bytes; nt <= 131072.
+ */
+int64_t gtrap_peakstats_ws_bytes(int32_t nt, int32_t nblocks);
+int gtrap_peakstats(double *st, int32_t *cnt, int32_t *top, const float *sn, const float *t0,
+                    int32_t nt, int32_t nq, int32_t mpd, int32_t ntop, void *ws,
+                    int64_t ws_bytes, int32_t nblocks, void *stream);
+
 /* ---- image assembly ---------------------------------------------------- */
 
 /*
