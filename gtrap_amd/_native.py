@@ -31,6 +31,8 @@ SIGNATURES: dict[str, tuple] = {
     "gtrap_inject_trapezoid": (_int, [_vp, _vp, _i32, _i32] + [_vp] * 5 + [_int, _vp]),
     "gtrap_peakstats_ws_bytes": (_i64, [_i32, _i32]),
     "gtrap_peakstats": (_int, [_vp] * 5 + [_i32] * 4 + [_vp, _i64, _i32, _vp]),
+    "gtrap_cutout": (_int, [_vp] * 6 + [_i32, _i32] + [_vp] * 3 + [_i32, _f64, _i32, _f64, _i32,
+                                                                      _vp]),
     "gtrap_blsstats": (_int, [_vp, _vp, _vp, _u32, _u32, _u32, _f64, _f64, _vp, _vp, _vp, _vp,
                               _vp, _vp]),
     "gtrap_gtls": (_int, [_vp] * 7 + [_i32] * 5 + [_f32] * 5 + [_int, _vp]),

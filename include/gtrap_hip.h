@@ -145,6 +145,22 @@ int gtrap_peakstats(double *st, int32_t *cnt, int32_t *top, const float *sn, con
                     int32_t nt, int32_t nq, int32_t mpd, int32_t ntop, void *ws,
                     int64_t ws_bytes, int32_t nblocks, void *stream);
 
+/* ---- candidate cut-outs ----------------------------------------------------- */
+
+/*
+ * Replaces gtrap/picktrap.py:27-139 (pick_Wnormalized_cleaned_lc_direct) for
+ * a batch of candidates c: curve cq[c], epoch ct0[c], width cw[c] on the
+ * interleaved (n, nq) lc/tu image.  Output per candidate: lcsx[c*nx + i],
+ * igapx[c*nx + i] (float64), prec[c] (1/0) and status[c] (0 ok, 1 no valid
+ * time >= T0, 2 gap cleaning stuck, 3 window shorter than 2 slots, 4 window
+ * longer than 2048 slots, 5 curve index outside [0, nq); not ok -> all ones,
+ * as the reference's fallback).
+ */
+int gtrap_cutout(double *lcsx, double *igapx, int32_t *prec, int32_t *status, const float *lc,
+                 const float *tu, int32_t n, int32_t nq, const int32_t *cq, const double *ct0,
+                 const double *cw, int32_t ncand, double alpha, int32_t nx, double daytopix,
+                 int32_t contcrit, void *stream);
+
 /* ---- image assembly ---------------------------------------------------- */
 
 /*
