@@ -135,19 +135,29 @@ struct TfSums {  // sums over a sample range in epoch coordinates
 // cM = W/2 (<=).  Returns S/N; writes H.
 // Region sums of one trial: counts n*, flux sums s* (x' = x - x_ref), and for
 // the ingress/egress regions B and D the sums of d, d^2 and x'd (d = offset
-// from the epoch); the window's count, x' and x'^2 sums; the count and x' sum
-// of the closed window (for the mean).
+// from the epoch); the window's mean of x' over [-W/2, W/2] and its sum of
+// (x' - mean)^2 over [-W/2, W/2) (both per (epoch, W), shared by every L).
 struct TfRegions {
   double nA, nB, nC, nD, nE, sA, sB, sC, sD, sE;
   double dB, dD, d2B, d2D, xdB, xdD;
-  double nAll, s1All, s2All, mcnt, sM;
+  double mean, y2;
 };
 
+// Window statistics of an (epoch, W): mean over the closed window and the
+// sum of squares of x' - mean over [-W/2, W/2) (tls.py:187 and the residual's
+// constant term).
+__device__ __forceinline__ void tf_window(double mcnt, double sM, double nAll, double s1All,
+                                          double s2All, double &mean, double &y2) {
+  mean = sM / mcnt;
+  y2 = s2All - 2.0 * mean * s1All + mean * mean * nAll;
+}
+
 // The statistic of one trial from its region sums (tls.py:44-79 expanded):
-// mean over [-W/2, W/2], H = -numB/denA, residual expanded in the sums,
-// S/N = -H / sqrt(res / (n - 3)).  Writes H.
-__device__ __forceinline__ double tf_eval(const TfRegions &g, double W, double L, double *Hout) {
-  const double mean = g.sM / g.mcnt;
+// H = -numB/denA, residual expanded in the sums, S/N = -H / sqrt(res/(n-3)).
+// invL = 1/L (infinite for L = 0, whose trial is NaN).  Writes H.
+__device__ __forceinline__ double tf_eval(const TfRegions &g, double W, double L, double invL,
+                                          double *Hout) {
+  const double mean = g.mean;
   const double yA = g.sA - mean * g.nA, yB = g.sB - mean * g.nB, yC = g.sC - mean * g.nC,
                yD = g.sD - mean * g.nD, yE = g.sE - mean * g.nE;
   const double ydB = g.xdB - mean * g.dB, ydD = g.xdD - mean * g.dD;
@@ -157,11 +167,9 @@ __device__ __forceinline__ double tf_eval(const TfRegions &g, double W, double L
   const double numB = 8.0 * L * L * xAEC - 4.0 * W * L * xBD + 16.0 * L * (ydD - ydB);
   const double H = -numB / denA;
   const double h = H / 2.0;
-  const double a = H / L;
-  // sum of y^2 over A..E: samples in [-W/2, W/2)
-  const double y2 = g.s2All - 2.0 * mean * g.s1All + mean * mean * g.nAll;
+  const double a = H * invL;
   const double q4 = W / 4.0;
-  double res = y2 + 2.0 * h * (yA - yC + yE) + h * h * (g.nA + g.nC + g.nE);
+  double res = g.y2 + 2.0 * h * (yA - yC + yE) + h * h * (g.nA + g.nC + g.nE);
   res += -2.0 * a * (ydB + q4 * yB) + a * a * (g.d2B + 2.0 * q4 * g.dB + q4 * q4 * g.nB);
   res += 2.0 * a * (ydD - q4 * yD) + a * a * (g.d2D - 2.0 * q4 * g.dD + q4 * q4 * g.nD);
   const double nn = g.nA + g.nB + g.nC + g.nD + g.nE;
@@ -180,10 +188,10 @@ __device__ __forceinline__ double tf_eval(const TfRegions &g, double W, double L
 __device__ __forceinline__ double tf_stat(const TfSums &c0, const TfSums &cAB,
                                           const TfSums &cBC, const TfSums &cCD,
                                           const TfSums &cDE, const TfSums &cE,
-                                          const TfSums &cM, double W, double L, double *Hout) {
+                                          const TfSums &cM, double W, double L, double invL,
+                                          double *Hout) {
   TfRegions g;
-  g.mcnt = cM.n - c0.n;
-  g.sM = cM.s1 - c0.s1;
+  tf_window(cM.n - c0.n, cM.s1 - c0.s1, cE.n - c0.n, cE.s1 - c0.s1, cE.s2 - c0.s2, g.mean, g.y2);
   g.nA = cAB.n - c0.n; g.nB = cBC.n - cAB.n; g.nC = cCD.n - cBC.n;
   g.nD = cDE.n - cCD.n; g.nE = cE.n - cDE.n;
   g.sA = cAB.s1 - c0.s1; g.sB = cBC.s1 - cAB.s1; g.sC = cCD.s1 - cBC.s1;
@@ -191,10 +199,7 @@ __device__ __forceinline__ double tf_stat(const TfSums &c0, const TfSums &cAB,
   g.dB = cBC.d1 - cAB.d1; g.dD = cDE.d1 - cCD.d1;
   g.d2B = cBC.d2 - cAB.d2; g.d2D = cDE.d2 - cCD.d2;
   g.xdB = cBC.xd - cAB.xd; g.xdD = cDE.xd - cCD.xd;
-  g.nAll = cE.n - c0.n;
-  g.s1All = cE.s1 - c0.s1;
-  g.s2All = cE.s2 - c0.s2;
-  return tf_eval(g, W, L, Hout);
+  return tf_eval(g, W, L, invL, Hout);
 }
 
 struct TfArgs {
@@ -221,10 +226,12 @@ struct TfArgs {
   const int64_t *item_pidx;
   const int32_t *nitems_dev;
   int32_t *counter;
+  int32_t te_run;  // epochs per window-table run (fits the dynamic LDS)
 };
 
 // LDS layout for a staged range
-constexpr int kTfMaxTe = 48;     // epochs per window-table run
+constexpr int kTfMaxTe = 48;     // epochs per window-table run (at most)
+constexpr int kTfWl = 6;         // (W, L) table row: tAB, tBC, tCD, tDE, L, 1/L
 constexpr int kTfMaxTile = 96;   // epochs per staged tile: with the reference at the
                                  // tile centre, |q| stays within 48 epoch steps + W/2
 constexpr int kTfMaxT = 4;    // chunked sweep: (epoch, W, boundary) sums per thread
@@ -256,7 +263,7 @@ __device__ __forceinline__ void tf_bnd(double W, const double *wrow, int b, doub
   if (b == 1) { c = 0.5 * W; kind = 0; return; }
   if (b == 2) { c = 0.5 * W; kind = 1; return; }
   const int m = (b - 3) >> 2, r = (b - 3) & 3;
-  c = wrow[5 * m + r];
+  c = wrow[kTfWl * m + r];
   kind = 0;
 }
 
@@ -428,13 +435,13 @@ __device__ __forceinline__ int tf_count(const TfLds &L, int a, int b, double t0,
 // raw differences first (x', x'^2 for the window and regions; q, q^2, x'q
 // only for B and D), shifted to epoch coordinates (d = q - s) per region.
 __device__ __forceinline__ double tf_stat_px(const TfLds &L, int p0, int pAB, int pBC, int pCD,
-                                             int pDE, int pE, int pM, double s, double W,
-                                             double Lw, double *Hout) {
+                                             int pDE, int pE, double mean, double y2, double s,
+                                             double W, double Lw, double invL, double *Hout) {
   const double x0 = L.px[p0][0], xAB = L.px[pAB][0], xBC = L.px[pBC][0], xCD = L.px[pCD][0],
-               xDE = L.px[pDE][0], xE = L.px[pE][0], xM = L.px[pM][0];
+               xDE = L.px[pDE][0], xE = L.px[pE][0];
   TfRegions g;
-  g.mcnt = (double)(pM - p0);
-  g.sM = xM - x0;
+  g.mean = mean;
+  g.y2 = y2;
   g.nA = (double)(pAB - p0); g.nB = (double)(pBC - pAB); g.nC = (double)(pCD - pBC);
   g.nD = (double)(pDE - pCD); g.nE = (double)(pE - pDE);
   g.sA = xAB - x0; g.sB = xBC - xAB; g.sC = xCD - xBC; g.sD = xDE - xCD; g.sE = xE - xDE;
@@ -448,10 +455,7 @@ __device__ __forceinline__ double tf_stat_px(const TfLds &L, int p0, int pAB, in
   g.dD = q1D - g.nD * s;
   g.d2D = q2D - 2.0 * s * q1D + g.nD * s * s;
   g.xdD = xqD - s * g.sD;
-  g.nAll = (double)(pE - p0);
-  g.s1All = xE - x0;
-  g.s2All = L.px[pE][1] - L.px[p0][1];
-  return tf_eval(g, W, Lw, Hout);
+  return tf_eval(g, W, Lw, invL, Hout);
 }
 
 static_assert(sizeof(TfLds) >= (kTfMaxBuckets + 1) * sizeof(int), "bucket counters exceed LDS");
@@ -472,14 +476,17 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
   const int nbnd = 3 + 4 * nl;
   // dynamic LDS: per-(W, L) region offsets and L, the widths, then the
   // per-(epoch, W) window table
-  double *wlt = reinterpret_cast<double *>(tf_dyn);  // [nw*nl][5]: tAB, tBC, tCD, tDE, L
-  double *sw = wlt + (size_t)5 * nw * nl;            // [nw]
-  int *ewp = reinterpret_cast<int *>(sw + nw);       // [kTfMaxTe*nw][3] window table
+  double *wlt = reinterpret_cast<double *>(tf_dyn);  // [nw*nl][kTfWl]
+  double *sw = wlt + (size_t)kTfWl * nw * nl;        // [nw]
+  double *ewd = sw + nw;                             // [te_run*nw][2]: window mean, y2
+  int *ewp = reinterpret_cast<int *>(ewd + (size_t)2 * A.te_run * nw);  // [te_run*nw][3]
   for (int k = tid; k < nw * nl; k += kTfThreads) {
     const int w = k / nl, m = k % nl;
     const double W = A.widths[w];
-    tf_bounds(W, m, nl, wlt[5 * k], wlt[5 * k + 1], wlt[5 * k + 2], wlt[5 * k + 3]);
-    wlt[5 * k + 4] = (double)m * (0.45 * W / (double)nl);
+    double *row = wlt + kTfWl * k;
+    tf_bounds(W, m, nl, row[0], row[1], row[2], row[3]);
+    row[4] = (double)m * (0.45 * W / (double)nl);
+    row[5] = 1.0 / row[4];
   }
   for (int w = tid; w < nw; w += kTfThreads) sw[w] = A.widths[w];
   __syncthreads();
@@ -695,9 +702,9 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
       if (cnt <= kTfChunk) {
         const double r = (double)((e0 + e1 - 1) / 2) * A.dtep;  // q reference: central epoch
         tf_stage(L, su, sx, nv, P, lo, cnt, r, xref);
-        // the staged tile is swept in runs of up to kTfMaxTe epochs
-        for (int eb = e0; eb < e1; eb += kTfMaxTe) {
-          const int nte = min(kTfMaxTe, e1 - eb);
+        // the staged tile is swept in runs of te_run epochs
+        for (int eb = e0; eb < e1; eb += A.te_run) {
+          const int nte = min(A.te_run, e1 - eb);
           // (epoch, W) window table: p0 (d >= -W/2), pE (d >= W/2), pM (d > W/2)
           for (int k = tid; k < nte * nw; k += kTfThreads) {
             const int ep = eb + k / nw, w = k % nw;
@@ -706,6 +713,9 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
             const int pE = tf_find(L, p0, cnt, t0, 0.5 * W, 0);
             const int pM = tf_find(L, pE, cnt, t0, 0.5 * W, 1);
             ewp[3 * k] = p0; ewp[3 * k + 1] = pE; ewp[3 * k + 2] = pM;
+            tf_window((double)(pM - p0), L.px[pM][0] - L.px[p0][0], (double)(pE - p0),
+                      L.px[pE][0] - L.px[p0][0], L.px[pE][1] - L.px[p0][1], ewd[2 * k],
+                      ewd[2 * k + 1]);
           }
           __syncthreads();
           // one thread per (epoch, W, L) trial, L fastest; the thread's
@@ -724,8 +734,8 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
             }
             const int j = je * nw + w, ep = eb + je;
             const double t0 = (double)ep * A.dtep, s = t0 - r, W = sw[w];
-            const int p0 = ewp[3 * j], pE = ewp[3 * j + 1], pM = ewp[3 * j + 2];
-            const double *bd = wlt + 5 * (w * nl + m);
+            const int p0 = ewp[3 * j], pE = ewp[3 * j + 1];
+            const double *bd = wlt + kTfWl * (w * nl + m);
             const double tAB = bd[0], tBC = bd[1], tCD = bd[2], tDE = bd[3];
             int aAB, bAB, aBC, bBC, aCD, bCD, aDE, bDE;
             tf_bracket(L, t0, tAB, aAB, bAB);
@@ -743,7 +753,8 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
             const int pCD = tf_clamp(rCD, pBC, pE);
             const int pDE = tf_clamp(rDE, pCD, pE);
             double H;
-            const double sn = tf_stat_px(L, p0, pAB, pBC, pCD, pDE, pE, pM, s, W, bd[4], &H);
+            const double sn = tf_stat_px(L, p0, pAB, pBC, pCD, pDE, pE, ewd[2 * j], ewd[2 * j + 1],
+                                         s, W, bd[4], bd[5], &H);
             if (tf_better(sn, w, m, ep, best)) best = TfBest{sn, H, w, m, ep};
           }
           __syncthreads();
@@ -775,7 +786,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
               const double t0 = (double)(e0 + el) * A.dtep;
               double cval;
               int kind;
-              tf_bnd(sw[w], wlt + 5 * w * nl, b, cval, kind);
+              tf_bnd(sw[w], wlt + kTfWl * w * nl, b, cval, kind);
               const int p = tf_find(L, 0, cc, t0, cval, kind);
               const TfSums cs = tf_cum(L, p, t0 - r);
               acc[j][0] += cs.n; acc[j][1] += cs.s1; acc[j][2] += cs.s2;
@@ -805,7 +816,8 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
           double H;
           const double sn = tf_stat(get(0), get(3 + 4 * m), get(4 + 4 * m), get(5 + 4 * m),
                                     get(6 + 4 * m), get(1), get(2), W,
-                                    wlt[5 * (w * nl + m) + 4], &H);
+                                    wlt[kTfWl * (w * nl + m) + 4], wlt[kTfWl * (w * nl + m) + 5],
+                                    &H);
           if (tf_better(sn, w, m, e0 + el, best)) best = TfBest{sn, H, w, m, e0 + el};
         }
         __syncthreads();
@@ -848,9 +860,32 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
 
 __global__ void tf_zero_ticket(int *p) { *p = 0; }
 
-static size_t tf_dyn_lds(int nw, int nl) {
-  return (size_t)5 * nw * nl * sizeof(double) + (size_t)nw * sizeof(double) +
-         (size_t)kTfMaxTe * nw * 3 * sizeof(int);
+// Dynamic LDS of a launch: the (W, L) table and widths, then te_run epochs
+// of window table (2 doubles + 3 ints per (epoch, W)), te_run as large as
+// the LDS left over by the static part allows (at most kTfMaxTe).
+static int tf_launch_cfg(int nw, int nl, size_t *dyn, int *te_run) {
+  static size_t static_lds = 0;
+  if (!static_lds) {
+    hipFuncAttributes fa;
+    if (hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(tlsfold_kernel<false>)) !=
+        hipSuccess) {
+      set_error("tlsfold: hipFuncGetAttributes failed");
+      return GTRAP_ELAUNCH;
+    }
+    static_lds = fa.sharedSizeBytes;
+  }
+  const size_t budget = 160 * 1024 > static_lds ? 160 * 1024 - static_lds : 0;
+  const size_t fixed = ((size_t)kTfWl * nw * nl + nw) * sizeof(double);
+  const size_t per = (size_t)nw * (2 * sizeof(double) + 3 * sizeof(int));
+  if (budget < fixed + per) {
+    set_error("tlsfold: nw x nl too large for LDS");
+    return GTRAP_EUNSUPPORTED;
+  }
+  int te = (int)((budget - fixed) / per);
+  if (te > kTfMaxTe) te = kTfMaxTe;
+  *te_run = te;
+  *dyn = fixed + (size_t)te * per;
+  return GTRAP_OK;
 }
 
 }  // namespace gtrap
@@ -886,14 +921,17 @@ extern "C" int gtrap_tlsfold(double *snr, double *hbest, int32_t *best_w, int32_
   GTRAP_REQUIRE(ws_bytes >= gtrap_tlsfold_ws_bytes(nmax, nblocks),
                 "gtrap_tlsfold: workspace too small");
   if (nptot == 0) return GTRAP_OK;
+  size_t dyn = 0;
+  int te_run = 0;
+  if (const int rc = gtrap::tf_launch_cfg(nw, nl, &dyn, &te_run)) return rc;
   gtrap::TfArgs A{t, x, off, len, periods, poff, nullptr, ncurves, nptot, widths, nw, nl, dtep,
                   snr, hbest, best_w, best_l, best_e, (unsigned char *)ws,
-                  tf_slot_bytes(nmax), nmax, nullptr, nullptr, nullptr, nullptr};
+                  tf_slot_bytes(nmax), nmax, nullptr, nullptr, nullptr, nullptr, te_run};
   const int64_t g = nptot < nblocks ? nptot : nblocks;
   hipStream_t st = gtrap::as_stream(stream);
   hipLaunchKernelGGL(gtrap::tf_zero_ticket, dim3(1), dim3(1), 0, st, (int *)ws);
   hipLaunchKernelGGL(gtrap::tlsfold_kernel<false>, dim3((unsigned)g), dim3(gtrap::kTfThreads),
-                     gtrap::tf_dyn_lds(nw, nl), st, A);
+                     dyn, st, A);
   return gtrap::check_launch("tlsfold_kernel");
 }
 
@@ -915,11 +953,14 @@ extern "C" int gtrap_tlsfold_items(double *snr, double *hbest, int32_t *best_w, 
   GTRAP_REQUIRE(dtep > 0.0 && nmax >= 1 && nblocks >= 1, "gtrap_tlsfold_items: bad scalars");
   GTRAP_REQUIRE(ws_bytes >= gtrap_tlsfold_ws_bytes(nmax, nblocks),
                 "gtrap_tlsfold_items: workspace too small");
+  size_t dyn = 0;
+  int te_run = 0;
+  if (const int rc = gtrap::tf_launch_cfg(nw, nl, &dyn, &te_run)) return rc;
   gtrap::TfArgs A{t, x, off, len, periods, nullptr, nullptr, 0, 0, widths, nw, nl, dtep,
                   snr, hbest, best_w, best_l, best_e, (unsigned char *)ws,
-                  tf_slot_bytes(nmax), nmax, item_curve, item_pidx, nitems_dev, counter};
+                  tf_slot_bytes(nmax), nmax, item_curve, item_pidx, nitems_dev, counter, te_run};
   hipLaunchKernelGGL(gtrap::tlsfold_kernel<true>, dim3((unsigned)nblocks), dim3(gtrap::kTfThreads),
-                     gtrap::tf_dyn_lds(nw, nl), gtrap::as_stream(stream), A);
+                     dyn, gtrap::as_stream(stream), A);
   return gtrap::check_launch("tlsfold_kernel<items>");
 }
 
