@@ -58,7 +58,7 @@ def bench_gfilter(steps: int, warmup: int):
     n, nq = 65536, 4096
     radii = (7, 24, 64)
     log(f"gfilter: generating {nq}x{n} image on device")
-    img = synth.config3(nq=nq, n=n, device="cuda")
+    img = synth.detrend_image(nq=nq, n=n, device="cuda")
     out = torch.empty_like(img)
     torch.cuda.synchronize()
     ev = {r: [] for r in radii}
@@ -122,8 +122,8 @@ def bench_geebls(steps: int, warmup: int):
     import torch
     from gtrap_amd import geebls, getstat, synth
     log("geebls: generating 1024 curves x 16384 samples")
-    d = synth.config2()
-    g = synth.config2_grid()
+    d = synth.bls_batch()
+    g = synth.bls_batch_grid()
     lc, tu, ntrue = d["lc"], d["tu"], d["ntrue"]
     n, nq = lc.shape
     nf = g["nf"]
@@ -196,7 +196,7 @@ def bench_serve(steps: int, warmup: int):
     from gtrap_amd.serve import ServerConfig, TransitServer, search_plain, PlainSearch
     cfg = ServerConfig()
     log(f"serve: building traces of {SERVE_REQUESTS} requests")
-    traces = [synth.config4_requests(SERVE_REQUESTS, seed=1004 + k)[0]
+    traces = [synth.serving_trace(SERVE_REQUESTS, seed=1004 + k)[0]
               for k in range(warmup + steps)]
     srv = TransitServer(cfg)
     srv.fold_log = []
@@ -300,7 +300,7 @@ def bench_precision(steps: int, warmup: int):
     from gtrap_amd import precision, synth
     nf = int(os.environ.get("GTRAP_BENCH_PREC_NF", "2048"))
     log(f"precision: generating 256 curves x 262144 samples, {nf} frequencies")
-    d = synth.config5(nf=nf)
+    d = synth.precision_batch(nf=nf)
     t0 = time.perf_counter()
     res = precision.study(d)
     wall = time.perf_counter() - t0

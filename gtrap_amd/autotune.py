@@ -125,8 +125,8 @@ if __name__ == "__main__":
     import sys
     from . import synth
     _native.require_gpu()
-    reqs, _ = synth.config4_requests(24, seed=7)
+    reqs, _ = synth.serving_trace(24, seed=7)
     print(tune_tlsfold_blocks([(r.t, r.flux) for r in reqs[:6]], [r.periods for r in reqs[:6]],
                               np.linspace(0.04, 0.3, 10)), file=sys.stderr)
     print(tune_medfilt_seg(65536, 256, 64), file=sys.stderr)
-    print(tune_serve(lambda: synth.config4_requests(48, seed=8)[0]), file=sys.stderr)
+    print(tune_serve(lambda: synth.serving_trace(48, seed=8)[0]), file=sys.stderr)

@@ -54,7 +54,7 @@ def to_f16(lc: np.ndarray, offset: np.ndarray):
 
 
 class Inputs:
-    """Device copies of a precision workload (synth.config5 layout)."""
+    """Device copies of a precision workload (synth.precision_batch layout)."""
 
     def __init__(self, d: dict):
         import torch

@@ -40,7 +40,7 @@ def box_dip(t, t0, period, depth, duration):
     return np.where(np.abs(dtc) < 0.5 * duration, depth, 0.0)
 
 
-def config1(seed: int = 1001):
+def tls_single_curve(seed: int = 1001):
     """One curve, 4096 samples at 0.0204 d, sigma 1e-3, one periodic trapezoid
     (depth 2e-3, P 3.7 d, T14 0.12 d, ingress 0.01 d, random epoch)."""
     rng = np.random.default_rng(seed)
@@ -56,7 +56,7 @@ def config1(seed: int = 1001):
                                                   depth=depth))
 
 
-def config2(nq: int = 1024, n: int = 16384, seed: int = 1002):
+def bls_batch(nq: int = 1024, n: int = 16384, seed: int = 1002):
     """Batched geebls workload: non-common time (1% cadence jitter per curve),
     5% of samples masked in 3 gaps (time set to INVAL, flux zig-zag filled
     with the curve's min/max as gtrap/read_keplerlc.py:39-46), AR(1) red noise
@@ -101,7 +101,7 @@ def config2(nq: int = 1024, n: int = 16384, seed: int = 1002):
     return dict(lc=lc, tu=tu, ntrue=ntrue, truth=truth)
 
 
-def config2_grid(n: int = 16384):
+def bls_batch_grid(n: int = 16384):
     """Trial grid of the batched geebls workload: fmin = 1/30, nf = 2048
     frequencies up to 1 d^-1, nb = 1024 bins, widths kmi < k <= kma with
     kmi = 10, kma = 22 (12 widths), kkmi = int(n*kmi/nb), block 512."""
@@ -113,7 +113,7 @@ def config2_grid(n: int = 16384):
                 nthread=512)
 
 
-def config3(nq: int = 4096, n: int = 65536, seed: int = 1003, device=None):
+def detrend_image(nq: int = 4096, n: int = 65536, seed: int = 1003, device=None):
     """gfilter workload: 1 + quadratic trend + sinusoid + noise 1e-3 + 0.1%
     outliers of +-1e-2.  With ``device`` the image is generated with torch on
     that device (same distribution, torch's generator)."""
@@ -153,7 +153,7 @@ def config3(nq: int = 4096, n: int = 65536, seed: int = 1003, device=None):
     return f.astype(np.float32)
 
 
-def config5(nq: int = 256, n: int = 262144, seed: int = 1005, period: float = 7.3,
+def precision_batch(nq: int = 256, n: int = 262144, seed: int = 1005, period: float = 7.3,
             nf: int = 128):
     """Precision workload: nq curves of n samples at a 2-minute cadence (364 d
     for n = 262144), one periodic trapezoid per curve at P = ``period``
@@ -186,7 +186,7 @@ def config5(nq: int = 256, n: int = 262144, seed: int = 1005, period: float = 7.
                 truth=dict(P=period, snr=snr, q=qd, depth=depth))
 
 
-def config4_requests(count: int = 200, seed: int = 1004, rate: float = 2000.0,
+def serving_trace(count: int = 200, seed: int = 1004, rate: float = 2000.0,
                      nlo: int = 2048, nhi: int = 70000, period_stride: int = 1):
     """Serving trace: Poisson arrivals (mean ``rate`` req/s), lengths
     log-uniform nlo..nhi at 0.0204 d cadence, flux = 1 + slow trend + noise

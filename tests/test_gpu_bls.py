@@ -20,7 +20,7 @@ def _dev(a):
 
 def _small(n=2048, nq=5, seed=0):
     from gtrap_amd import synth
-    d = synth.config2(nq=nq, n=n, seed=seed)
+    d = synth.bls_batch(nq=nq, n=n, seed=seed)
     lc, tu, ntrue = d["lc"], d["tu"], d["ntrue"]
     off = np.where(tu >= 0, lc, 0).astype(np.float64).sum(0) / n
     return lc, tu, off.astype(np.float32), ntrue
@@ -108,14 +108,14 @@ def test_geebls_dropin_module_and_getstat(gpu):
     np.testing.assert_array_equal(pb2, op2)
 
 
-def test_config2_full_batch_subset(gpu):
+def test_bls_batch_full_size_subset(gpu):
     """1024 curves x 16384 samples, 2048 frequencies, nb 1024, kma 22: the whole
     batch on the GPU, a subset of (curve, frequency) cells against the oracle,
     and box recovery on the injected curves."""
     import torch
     from gtrap_amd import geebls, getstat, synth
-    d = synth.config2()
-    g = synth.config2_grid()
+    d = synth.bls_batch()
+    g = synth.bls_batch_grid()
     lc, tu, ntrue = d["lc"], d["tu"], d["ntrue"]
     n, nq = lc.shape
     off = (np.where(tu >= 0, lc, 0).astype(np.float64).sum(0) / n).astype(np.float32)

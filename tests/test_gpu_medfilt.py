@@ -109,13 +109,13 @@ def test_get_offset(gpu):
     np.testing.assert_array_equal(ntrue.cpu().numpy(), (tu >= 0).sum(0))
 
 
-def test_config3_full_size_subset(gpu):
+def test_detrend_image_full_size_subset(gpu):
     """4096 curves x 65536 samples, r in {7, 24, 64}: 16 random curves vs scipy,
     div mode (the workload's output is flux / median)."""
     import torch
     from gtrap_amd import gfilter, synth
     n, nq = 65536, 4096
-    img = synth.config3(nq=nq, n=n, device="cuda")
+    img = synth.detrend_image(nq=nq, n=n, device="cuda")
     out = torch.empty_like(img)
     rng = np.random.default_rng(11)
     cols = rng.choice(nq, 16, replace=False)

@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 
 def _data(seed=11):
     from gtrap_amd import synth
-    return synth.config5(nq=6, n=32768, nf=24, seed=seed)
+    return synth.precision_batch(nq=6, n=32768, nf=24, seed=seed)
 
 
 def test_default_variant_equals_geebls(gpu):

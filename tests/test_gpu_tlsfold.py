@@ -67,14 +67,14 @@ def test_single_epoch_limit_and_nl_grid(gpu):
     _check([(t, x)], [np.array([2.5 * base, 4.0 * base])], widths, 4)
 
 
-def test_config1_grid_matches_oracle(gpu):
-    """Config-1 curve (4096 samples, trapezoid at P = 3.7 d) on every 20th
+def test_single_curve_grid_matches_oracle(gpu):
+    """The single-curve workload (4096 samples, trapezoid at P = 3.7 d) on every 20th
     period of the 1000-period grid plus the true period: the GPU's per-period
     best trials equal the direct oracle's.  (tls.py's S/N is H over the
     per-sample residual RMS, so the periodogram maximum need not sit at the
     injected period; the comparison is with the oracle, not the injection.)"""
     from gtrap_amd import synth, tlsfold
-    d = synth.config1()
+    d = synth.tls_single_curve()
     per = np.sort(np.concatenate([d["periods"][::20], [d["truth"]["P"]]]))
     _check([(d["t"], d["flux"])], [per], d["widths"], 10)
     got = tlsfold.tlsfold([(d["t"], d["flux"])], [per], d["widths"], 10, 0.0204)[0]
