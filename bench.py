@@ -194,8 +194,9 @@ def bench_serve(steps: int, warmup: int):
     import torch
     from gtrap_amd import synth
     from gtrap_amd.serve import ServerConfig, TransitServer, search_plain, PlainSearch
-    cfg = ServerConfig()
-    log(f"serve: building traces of {SERVE_REQUESTS} requests")
+    cfg = ServerConfig.tuned()
+    log(f"serve: building traces of {SERVE_REQUESTS} requests; items/step {cfg.items_per_step}, "
+        f"workgroups {cfg.nblocks}")
     traces = [synth.serving_trace(SERVE_REQUESTS, seed=1004 + k)[0]
               for k in range(warmup + steps)]
     srv = TransitServer(cfg)

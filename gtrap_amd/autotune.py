@@ -7,7 +7,10 @@ grids, windows) are never touched, so every tuned configuration returns the
 same bits.
 
 Results are cached as JSON keyed by device name and workload signature in
-``$GTRAP_TUNE_CACHE`` (default ``~/.cache/gtrap_amd/tune.json``).
+``$GTRAP_TUNE_CACHE`` (default ``~/.cache/gtrap_amd/tune.json``).  Lookups
+fall back to the table shipped with the package (``tuned.json``, written by
+``python -m gtrap_amd.autotune`` on the target GPU), then to the caller's
+default.
 """
 from __future__ import annotations
 
@@ -46,8 +49,18 @@ def store(key: str, value) -> None:
     p.write_text(json.dumps(d, indent=1, sort_keys=True))
 
 
+PACKAGED = Path(__file__).resolve().with_name("tuned.json")
+
+
 def lookup(key: str, default=None):
-    return load().get(f"{_device_key()}:{key}", default)
+    k = f"{_device_key()}:{key}"
+    v = load().get(k)
+    if v is None:
+        try:
+            v = json.loads(PACKAGED.read_text()).get(k)
+        except (OSError, ValueError):
+            v = None
+    return default if v is None else v
 
 
 def _time(fn, reps=3) -> float:
@@ -121,12 +134,30 @@ def tune_serve(trace_fn, candidates=((16384, 1024), (32768, 1024), (65536, 1024)
     return dict(items_per_step=ips, nblocks=nb, rates=rates)
 
 
-if __name__ == "__main__":
-    import sys
+def main(argv=None) -> int:
+    """Tune the serving step and the fold kernel's workgroups on the current
+    GPU; prints the choices and stores them in the cache."""
+    import argparse
     from . import synth
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--requests", type=int, default=48)
+    args = ap.parse_args(argv)
     _native.require_gpu()
     reqs, _ = synth.serving_trace(24, seed=7)
-    print(tune_tlsfold_blocks([(r.t, r.flux) for r in reqs[:6]], [r.periods for r in reqs[:6]],
-                              np.linspace(0.04, 0.3, 10)), file=sys.stderr)
-    print(tune_medfilt_seg(65536, 256, 64), file=sys.stderr)
-    print(tune_serve(lambda: synth.serving_trace(48, seed=8)[0]), file=sys.stderr)
+    out = dict(device=_device_key())
+    out["tlsfold"] = tune_tlsfold_blocks([(r.t, r.flux) for r in reqs[:6]],
+                                         [r.periods for r in reqs[:6]],
+                                         np.linspace(0.04, 0.3, 10),
+                                         candidates=(256, 512, 1024, 2048))
+    print(json.dumps(out["tlsfold"]), flush=True)
+    out["serve"] = tune_serve(lambda: synth.serving_trace(args.requests, seed=8)[0],
+                              candidates=((16384, 512), (32768, 512), (65536, 512),
+                                          (16384, 1024), (32768, 1024), (65536, 1024),
+                                          (131072, 1024)))
+    print(json.dumps(out["serve"]), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    import sys
+    sys.exit(main())
