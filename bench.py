@@ -240,7 +240,9 @@ def bench_serve(steps: int, warmup: int):
     # every launch of the run (warm-up included, as a profiler sees them)
     # bracketed by HIP events on the serving stream
     nlaunch, kms, ktr, ksec = srv.fold_stats()
-    ach = TRIAL_FLOPS * ktr / ksec / 1e12
+    # trials with L = 0 are NaN by construction and are not evaluated
+    evaluated = ktr * (cfg.nl - 1) / cfg.nl
+    ach = TRIAL_FLOPS * evaluated / ksec / 1e12
     # CPU baseline: the direct oracle on a bounded sample of periods
     from oracle import tlsfold as ot
     from oracle import medfilt as om
@@ -282,6 +284,7 @@ def bench_serve(steps: int, warmup: int):
                                      "(mean bytes per launch, copy-calibrated)",
                       kernel="tlsfold_kernel<true> (serving step)", launches=nlaunch,
                       kernel_ms=kms, trials_per_launch=ktr / max(nlaunch, 1),
+                      evaluated_trials_per_launch=evaluated / max(nlaunch, 1),
                       flops_per_trial=TRIAL_FLOPS),
         cpu_baseline=dict(value=cpu_lcs, unit="light curves/s", cores=1, kind="port",
                           trial_evals_per_s=cpu_tps,

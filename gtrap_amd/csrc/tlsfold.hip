@@ -256,13 +256,13 @@ struct TfLds {
 
 // boundary b of width W (kind, offset) from the width's row of the (W, L)
 // table: 0 -> -W/2 (>=), 1 -> W/2 (>=, region end), 2 -> W/2 (>, mean end),
-// 3 + 4m + {0,1,2,3} -> tAB, tBC, tCD, tDE of step m (>=)
+// 3 + 4(m-1) + {0,1,2,3} -> tAB, tBC, tCD, tDE of step m >= 1 (>=)
 __device__ __forceinline__ void tf_bnd(double W, const double *wrow, int b, double &c,
                                        int &kind) {
   if (b == 0) { c = -0.5 * W; kind = 0; return; }
   if (b == 1) { c = 0.5 * W; kind = 0; return; }
   if (b == 2) { c = 0.5 * W; kind = 1; return; }
-  const int m = (b - 3) >> 2, r = (b - 3) & 3;
+  const int m = ((b - 3) >> 2) + 1, r = (b - 3) & 3;
   c = wrow[kTfWl * m + r];
   kind = 0;
 }
@@ -473,7 +473,10 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
   extern __shared__ __align__(16) unsigned char tf_dyn[];
   const int tid = threadIdx.x;
   const int nw = A.nw, nl = A.nl;
-  const int nbnd = 3 + 4 * nl;
+  // L = 0 (m = 0) empties the ingress/egress regions: denA = numB = 0 and the
+  // trial's S/N is NaN, which never wins, so only m = 1..nl-1 are evaluated
+  const int nlt = nl - 1;
+  const int nbnd = 3 + 4 * nlt;
   // dynamic LDS: per-(W, L) region offsets and L, the widths, then the
   // per-(epoch, W) window table
   double *wlt = reinterpret_cast<double *>(tf_dyn);  // [nw*nl][kTfWl]
@@ -491,7 +494,8 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
   for (int w = tid; w < nw; w += kTfThreads) sw[w] = A.widths[w];
   __syncthreads();
   // advance of the trial loop's (L, W, epoch) indices per kTfThreads trials
-  const int dm = kTfThreads % nl, dw = (kTfThreads / nl) % nw, dje = kTfThreads / (nl * nw);
+  const int dm = nlt ? kTfThreads % nlt : 0, dw = nlt ? (kTfThreads / nlt) % nw : 0,
+            dje = nlt ? kTfThreads / (nlt * nw) : 0;
   unsigned char *ws = A.ws + 256 + (size_t)blockIdx.x * (size_t)A.ws_per_block;
   double *su = reinterpret_cast<double *>(ws);
   float *sx = reinterpret_cast<float *>(su + A.nmax);
@@ -718,21 +722,22 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
                       ewd[2 * k + 1]);
           }
           __syncthreads();
-          // one thread per (epoch, W, L) trial, L fastest; the thread's
+          // one thread per (epoch, W, L >= 1) trial, L fastest; the thread's
           // (L, W, epoch) indices advance by kTfThreads trials per step
-          const int ntr = nte * nw * nl;
-          int m = tid % nl, w = (tid / nl) % nw, je = tid / (nl * nw);
+          const int ntr = nte * nw * nlt;
+          int mm = nlt ? tid % nlt : 0, w = nlt ? (tid / nlt) % nw : 0,
+              je = nlt ? tid / (nlt * nw) : 0;
           for (int k = tid; k < ntr; k += kTfThreads) {
             if (k != tid) {
-              m += dm;
-              const int c1 = m >= nl ? 1 : 0;
-              m -= c1 * nl;
+              mm += dm;
+              const int c1 = mm >= nlt ? 1 : 0;
+              mm -= c1 * nlt;
               w += dw + c1;
               const int c2 = w >= nw ? 1 : 0;
               w -= c2 * nw;
               je += dje + c2;
             }
-            const int j = je * nw + w, ep = eb + je;
+            const int j = je * nw + w, ep = eb + je, m = mm + 1;
             const double t0 = (double)ep * A.dtep, s = t0 - r, W = sw[w];
             const int p0 = ewp[3 * j], pE = ewp[3 * j + 1];
             const double *bd = wlt + kTfWl * (w * nl + m);
@@ -804,8 +809,8 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
             for (int v = 0; v < 6; ++v) accl[6 * k + v] = acc[j][v];
         }
         __syncthreads();
-        for (int k = tid; k < tce * nw * nl; k += kTfThreads) {
-          const int m = k % nl, jj = k / nl;
+        for (int k = tid; k < tce * nw * nlt; k += kTfThreads) {
+          const int mm = k % nlt, jj = k / nlt, m = mm + 1;
           const int el = jj / nw, w = jj % nw;
           const double W = sw[w];
           const double *base = accl + 6 * ((size_t)(el * nw + w) * nbnd);
@@ -814,8 +819,8 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
             return TfSums{v[0], v[1], v[2], v[3], v[4], v[5]};
           };
           double H;
-          const double sn = tf_stat(get(0), get(3 + 4 * m), get(4 + 4 * m), get(5 + 4 * m),
-                                    get(6 + 4 * m), get(1), get(2), W,
+          const double sn = tf_stat(get(0), get(3 + 4 * mm), get(4 + 4 * mm), get(5 + 4 * mm),
+                                    get(6 + 4 * mm), get(1), get(2), W,
                                     wlt[kTfWl * (w * nl + m) + 4], wlt[kTfWl * (w * nl + m) + 5],
                                     &H);
           if (tf_better(sn, w, m, e0 + el, best)) best = TfBest{sn, H, w, m, e0 + el};
