@@ -30,8 +30,12 @@ def cache_path() -> Path:
 
 
 def _device_key() -> str:
+    """Architecture and CU count (e.g. ``gfx950_256cu``): the marketing name
+    depends on whether the runtime finds its id table, so it is not a key."""
     import torch
-    return torch.cuda.get_device_name(0).replace(" ", "_")
+    prop = torch.cuda.get_device_properties(0)
+    arch = getattr(prop, "gcnArchName", "") or torch.cuda.get_device_name(0)
+    return f"{arch.split(':')[0].replace(' ', '_')}_{prop.multi_processor_count}cu"
 
 
 def load() -> dict:
