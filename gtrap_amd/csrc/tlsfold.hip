@@ -53,15 +53,18 @@ __device__ __forceinline__ int tf_xkey(float x) {
   return b ^ ((b >> 31) & 0x7fffffff);
 }
 
+// A trial's rank key is k = sign(S/N) * (S/N)^2, a monotone function of the
+// S/N, so trials compare by key; the S/N and H of a thread's best trial come
+// from its key and its numerator and denominator of H (tf_sn, tf_h).
 struct TfBest {
-  double sn, h;
+  double key, numB, denA;
   int w, l, e;
 };
 
-// better: larger S/N, ties to the earliest (w, l, e); w < 0 means "none yet"
-__device__ __forceinline__ bool tf_better(double sn, int w, int l, int e, const TfBest &b) {
-  if (sn > b.sn) return true;
-  if (sn == b.sn && b.w >= 0)
+// better: larger key, ties to the earliest (w, l, e); w < 0 means "none yet"
+__device__ __forceinline__ bool tf_better(double key, int w, int l, int e, const TfBest &b) {
+  if (key > b.key) return true;
+  if (key == b.key && b.w >= 0)
     return (w < b.w) || (w == b.w && (l < b.l || (l == b.l && e < b.e)));
   return false;
 }
@@ -153,10 +156,13 @@ __device__ __forceinline__ void tf_window(double mcnt, double sM, double nAll, d
 }
 
 // The statistic of one trial from its region sums (tls.py:44-79 expanded):
-// H = -numB/denA, residual expanded in the sums, S/N = -H / sqrt(res/(n-3)).
-// invL = 1/L (infinite for L = 0, whose trial is NaN).  Writes H.
+// H = -numB/denA and the residual res = y2 + H U + H^2 V expanded in the
+// sums, so res denA^2 = y2 denA^2 - numB denA U + numB^2 V, and
+// S/N = -H / sqrt(res/(n-3)) has the rank key
+// k = sign(numB denA) numB^2 (n-3) / (res denA^2): one division per trial.
+// invL = 1/L.  Writes numB and denA; NaN for no trial (n <= 3, denA = 0).
 __device__ __forceinline__ double tf_eval(const TfRegions &g, double W, double L, double invL,
-                                          double *Hout) {
+                                          double *numBout, double *denAout) {
   const double mean = g.mean;
   const double yA = g.sA - mean * g.nA, yB = g.sB - mean * g.nB, yC = g.sC - mean * g.nC,
                yD = g.sD - mean * g.nD, yE = g.sE - mean * g.nE;
@@ -165,22 +171,36 @@ __device__ __forceinline__ double tf_eval(const TfRegions &g, double W, double L
   const double denA = 4.0 * L * L * (g.nA - g.nC + g.nE) + W * W * (g.nB + g.nD) +
                       8.0 * W * (g.dB - g.dD) + 16.0 * (g.d2B + g.d2D);
   const double numB = 8.0 * L * L * xAEC - 4.0 * W * L * xBD + 16.0 * L * (ydD - ydB);
-  const double H = -numB / denA;
-  const double h = H / 2.0;
-  const double a = H * invL;
   const double q4 = W / 4.0;
-  double res = g.y2 + 2.0 * h * (yA - yC + yE) + h * h * (g.nA + g.nC + g.nE);
-  res += -2.0 * a * (ydB + q4 * yB) + a * a * (g.d2B + 2.0 * q4 * g.dB + q4 * q4 * g.nB);
-  res += 2.0 * a * (ydD - q4 * yD) + a * a * (g.d2D - 2.0 * q4 * g.dD + q4 * q4 * g.nD);
+  // res = y2 + 2h(yA - yC + yE) + h^2(nA + nC + nE) - 2a(ydB + q4 yB)
+  //       + a^2(d2B + 2 q4 dB + q4^2 nB) + 2a(ydD - q4 yD) + a^2(d2D - 2 q4 dD + q4^2 nD),
+  // h = H/2, a = H/L
+  const double U = (yA - yC + yE) + 2.0 * invL * ((ydD - q4 * yD) - (ydB + q4 * yB));
+  const double V = 0.25 * (g.nA + g.nC + g.nE) +
+                   invL * invL * ((g.d2B + 2.0 * q4 * g.dB + q4 * q4 * g.nB) +
+                                  (g.d2D - 2.0 * q4 * g.dD + q4 * q4 * g.nD));
+  const double R = g.y2 * denA * denA - numB * denA * U + numB * numB * V;
   const double nn = g.nA + g.nB + g.nC + g.nD + g.nE;
-  *Hout = H;
+  *numBout = numB;
+  *denAout = denA;
+  constexpr double kNaN = __builtin_nan("");
   // n - 3 degrees of freedom: fewer than 4 samples is no trial (the direct
-  // formula gives 0/0 there, the expanded residual only rounding noise)
-  if (nn <= 3.0) return __longlong_as_double(0x7ff8000000000000LL);
-  // the expanded residual of a perfect fit can round below zero; a NaN
-  // residual (H = +-inf when denA == 0) must stay NaN, so no fmax here
-  return -H / sqrt((res < 0.0 ? 0.0 : res) / (nn - 3.0));
+  // formula gives 0/0 there); denA = 0 makes H infinite and the S/N NaN
+  if (nn <= 3.0 || denA == 0.0) return kNaN;
+  const double s2 = numB * numB * (nn - 3.0);
+  const bool neg = (numB < 0.0) != (denA < 0.0);
+  // the expanded residual of a perfect fit can round below zero: a residual
+  // of 0 gives an infinite S/N (0/0, NaN, when H = 0 too)
+  if (R <= 0.0) return numB == 0.0 ? kNaN : (neg ? -INFINITY : INFINITY);
+  const double k = s2 / R;
+  return neg ? -k : k;
 }
+
+// S/N and H of a best trial from its key and H's numerator and denominator
+__device__ __forceinline__ double tf_sn(double key) {
+  return key >= 0.0 ? sqrt(key) : -sqrt(-key);
+}
+__device__ __forceinline__ double tf_h(double numB, double denA) { return -numB / denA; }
 
 // The statistic from cumulative sums at the trial's boundaries (cumulative
 // from the window start, epoch coordinates): c0 = -W/2, cAB..cDE, cE = W/2
@@ -189,7 +209,7 @@ __device__ __forceinline__ double tf_stat(const TfSums &c0, const TfSums &cAB,
                                           const TfSums &cBC, const TfSums &cCD,
                                           const TfSums &cDE, const TfSums &cE,
                                           const TfSums &cM, double W, double L, double invL,
-                                          double *Hout) {
+                                          double *numB, double *denA) {
   TfRegions g;
   tf_window(cM.n - c0.n, cM.s1 - c0.s1, cE.n - c0.n, cE.s1 - c0.s1, cE.s2 - c0.s2, g.mean, g.y2);
   g.nA = cAB.n - c0.n; g.nB = cBC.n - cAB.n; g.nC = cCD.n - cBC.n;
@@ -199,7 +219,7 @@ __device__ __forceinline__ double tf_stat(const TfSums &c0, const TfSums &cAB,
   g.dB = cBC.d1 - cAB.d1; g.dD = cDE.d1 - cCD.d1;
   g.d2B = cBC.d2 - cAB.d2; g.d2D = cDE.d2 - cCD.d2;
   g.xdB = cBC.xd - cAB.xd; g.xdD = cDE.xd - cCD.xd;
-  return tf_eval(g, W, L, invL, Hout);
+  return tf_eval(g, W, L, invL, numB, denA);
 }
 
 struct TfArgs {
@@ -232,6 +252,7 @@ struct TfArgs {
 // LDS layout for a staged range
 constexpr int kTfMaxTe = 48;     // epochs per window-table run (at most)
 constexpr int kTfWl = 6;         // (W, L) table row: tAB, tBC, tCD, tDE, L, 1/L
+constexpr int kTfCells = 2 * kTfChunk;  // phase index cells of a staged range (at most)
 constexpr int kTfMaxTile = 96;   // epochs per staged tile: with the reference at the
                                  // tile centre, |q| stays within 48 epoch steps + W/2
 constexpr int kTfMaxT = 4;    // chunked sweep: (epoch, W, boundary) sums per thread
@@ -244,13 +265,14 @@ constexpr int kTfMaxBreaks = 64;      // sort chunks per period
 struct TfLds {
   double lu[kTfChunk];          // staged images (exact)
   double px[kTfChunk + 1][5];   // exclusive prefix of x', x'^2, q, q^2, x'q
-  // phase index of the staged range: tab[g] = first k with cell(lu[k]) >= g,
-  // cell(v) = clamp(floor((v - tbase) * tinv), 0, tnc - 1), tab[tnc] = cnt
-  int tab[kTfChunk + 1];
+  // phase index of the staged range (two cells per sample): tab[g] = first k
+  // with cell(lu[k]) >= g, cell(v) = clamp(floor((v - tbase) * tinv), 0, tnc - 1),
+  // tab[tnc] = cnt
+  unsigned short tab[kTfCells + 1];
   double tbase, tinv;
   int tnc;
   double scan[kTfThreads / kWave][5];
-  double red_sn[kTfThreads / kWave], red_h[kTfThreads / kWave];
+  double red_key[kTfThreads / kWave], red_nb[kTfThreads / kWave], red_da[kTfThreads / kWave];
   int red_w[kTfThreads / kWave], red_l[kTfThreads / kWave], red_e[kTfThreads / kWave];
 };
 
@@ -316,10 +338,10 @@ __device__ void tf_stage(TfLds &L, const double *su, const float *sx, int nv, do
     for (int v = 0; v < 5; ++v) L.scan[wv][v] = inc[v];
   }
   __syncthreads();
-  {  // phase index: about one sample per cell; cells narrower than 1e-9 of
-     // the magnitudes (a range of near-equal phases) fall back to one cell
+  {  // phase index: two cells per sample; cells narrower than 1e-9 of the
+     // magnitudes (a range of near-equal phases) fall back to one cell
     const double base = cnt > 0 ? L.lu[0] : 0.0, top = cnt > 0 ? L.lu[cnt - 1] : 0.0;
-    const int nc = cnt < 1 ? 1 : (cnt > kTfChunk ? kTfChunk : cnt);
+    const int nc = cnt < 1 ? 1 : 2 * (cnt > kTfChunk ? kTfChunk : cnt);
     const double span = top - base;
     const double inv =
         span > 1e-9 * (1.0 + fabs(base) + fabs(top)) * (double)nc ? (double)nc / span : 0.0;
@@ -329,11 +351,11 @@ __device__ void tf_stage(TfLds &L, const double *su, const float *sx, int nv, do
       if (k < cnt) {
         const int ck = tf_cell(L.lu[k], base, inv, nc);
         const int cp = k == 0 ? -1 : tf_cell(L.lu[k - 1], base, inv, nc);
-        for (int g = cp + 1; g <= ck; ++g) L.tab[g] = k;
+        for (int g = cp + 1; g <= ck; ++g) L.tab[g] = (unsigned short)k;
       }
     }
     const int cl = cnt > 0 ? tf_cell(top, base, inv, nc) : -1;
-    for (int g = cl + 1 + tid; g <= nc; g += kTfThreads) L.tab[g] = cnt;
+    for (int g = cl + 1 + tid; g <= nc; g += kTfThreads) L.tab[g] = (unsigned short)cnt;
     if (tid == 0) { L.tbase = base; L.tinv = inv; L.tnc = nc; }
   }
   double run[5];
@@ -364,11 +386,16 @@ __device__ void tf_stage(TfLds &L, const double *su, const float *sx, int nv, do
   __syncthreads();
 }
 
+// prefix row p (p <= kTfChunk, so a 24-bit multiply forms its offset)
+__device__ __forceinline__ const double *tf_px(const TfLds &L, int p) {
+  return &L.px[0][0] + __umul24((unsigned)p, 5u);
+}
+
 // cumulative sums (epoch coordinates, shift s = t0 - r) at staged index p
 __device__ __forceinline__ TfSums tf_cum(const TfLds &L, int p, double s) {
   const double n = (double)p;
-  const double x1 = L.px[p][0], x2 = L.px[p][1], q1 = L.px[p][2], q2 = L.px[p][3],
-               xq = L.px[p][4];
+  const double *row = tf_px(L, p);
+  const double x1 = row[0], x2 = row[1], q1 = row[2], q2 = row[3], xq = row[4];
   TfSums o;
   o.n = n;
   o.s1 = x1;
@@ -388,20 +415,18 @@ __device__ __forceinline__ TfSums tf_cum(const TfLds &L, int p, double s) {
 // answer lies in [tab[g-1], tab[g+2]] (one whole cell of margin each side).
 // The result is clamped to [lo, hi], which equals the search restricted to
 // [lo, hi) (the test is monotone in the sorted images).
+// Both cases are index selects (no branch); g = tf_cell(t0 + c) is the
+// clamped floor of the same f.
 constexpr double kTfEdge = 1e-4;
 __device__ __forceinline__ void tf_bracket(const TfLds &L, double t0, double c, int &a, int &b) {
   const int nc = L.tnc;
   const double f = (t0 + c - L.tbase) * L.tinv;
   const double fl = floor(f), fr = f - fl;
-  if (fl >= 0.0 && fl <= (double)(nc - 1) && fr >= kTfEdge && fr <= 1.0 - kTfEdge) {
-    const int g = (int)fl;
-    a = L.tab[g];
-    b = L.tab[g + 1];
-  } else {
-    const int g = tf_cell(t0 + c, L.tbase, L.tinv, nc);
-    a = L.tab[g > 0 ? g - 1 : 0];
-    b = L.tab[g + 2 < nc ? g + 2 : nc];
-  }
+  const int gi = (int)fmin(fmax(fl, -1.0), (double)nc);  // NaN -> -1
+  const bool inner = (unsigned)gi < (unsigned)nc && fabs(fr - 0.5) <= 0.5 - kTfEdge;
+  const int g = min(max(gi, 0), nc - 1);
+  a = L.tab[inner ? gi : max(g - 1, 0)];
+  b = L.tab[inner ? gi + 1 : min(g + 2, nc)];
 }
 
 __device__ __forceinline__ int tf_clamp(int r, int lo, int hi) {
@@ -415,20 +440,31 @@ __device__ __forceinline__ int tf_find(const TfLds &L, int lo, int hi, double t0
   return tf_clamp(tf_search(L.lu, a, b, t0, c, kind), lo, hi);
 }
 
-// First index passing the offset test (kind 0, d >= c) in a bracket [a, b)
-// of at most kTfCount samples: a plus the number of failing samples, from
-// independent LDS reads (no loop, so several searches overlap).  Reads past
-// b stay inside lu and are masked.
-constexpr int kTfCount = 4;
-__device__ __forceinline__ int tf_count(const TfLds &L, int a, int b, double t0, double c) {
-  int r = a;
+// First index passing the offset test (kind 0, d >= c) in each of four
+// brackets [a[i], b[i]) of at most kTfCount samples, by a branch-free
+// bisection of fixed depth (steps 4, 2, 1) with the four searches in step:
+// [a[i], r[i]) holds only failing samples throughout, and a step advances
+// r[i] past samples inside the bracket only.  Indices stay inside lu.
+constexpr int kTfCount = 7;
+__device__ __forceinline__ void tf_count4(const TfLds &L, const int *a, const int *b, double t0,
+                                          const double *c, int *r) {
 #pragma unroll
-  for (int k = 0; k < kTfCount; ++k) {
-    const int idx = a + k < kTfChunk ? a + k : kTfChunk - 1;
-    const double d = L.lu[idx] - t0;
-    r += (a + k < b && !(d >= c)) ? 1 : 0;
+  for (int i = 0; i < 4; ++i) r[i] = a[i];
+#pragma unroll
+  for (int st = 4; st > 0; st >>= 1) {
+    double d[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int k = r[i] + st - 1;
+      d[i] = L.lu[min(k, kTfChunk - 1)] - t0;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int k = r[i] + st - 1;
+      const int fail = (int)(k < b[i]) & (int)!(d[i] >= c[i]);
+      r[i] += fail * st;
+    }
   }
-  return r;
 }
 
 // The statistic from the staged prefix sums at the trial's boundary indices:
@@ -436,26 +472,26 @@ __device__ __forceinline__ int tf_count(const TfLds &L, int a, int b, double t0,
 // only for B and D), shifted to epoch coordinates (d = q - s) per region.
 __device__ __forceinline__ double tf_stat_px(const TfLds &L, int p0, int pAB, int pBC, int pCD,
                                              int pDE, int pE, double mean, double y2, double s,
-                                             double W, double Lw, double invL, double *Hout) {
-  const double x0 = L.px[p0][0], xAB = L.px[pAB][0], xBC = L.px[pBC][0], xCD = L.px[pCD][0],
-               xDE = L.px[pDE][0], xE = L.px[pE][0];
+                                             double W, double Lw, double invL, double *numB,
+                                             double *denA) {
+  const double *r0 = tf_px(L, p0), *rAB = tf_px(L, pAB), *rBC = tf_px(L, pBC),
+               *rCD = tf_px(L, pCD), *rDE = tf_px(L, pDE), *rE = tf_px(L, pE);
+  const double x0 = r0[0], xAB = rAB[0], xBC = rBC[0], xCD = rCD[0], xDE = rDE[0], xE = rE[0];
   TfRegions g;
   g.mean = mean;
   g.y2 = y2;
   g.nA = (double)(pAB - p0); g.nB = (double)(pBC - pAB); g.nC = (double)(pCD - pBC);
   g.nD = (double)(pDE - pCD); g.nE = (double)(pE - pDE);
   g.sA = xAB - x0; g.sB = xBC - xAB; g.sC = xCD - xBC; g.sD = xDE - xCD; g.sE = xE - xDE;
-  const double q1B = L.px[pBC][2] - L.px[pAB][2], q2B = L.px[pBC][3] - L.px[pAB][3],
-               xqB = L.px[pBC][4] - L.px[pAB][4];
-  const double q1D = L.px[pDE][2] - L.px[pCD][2], q2D = L.px[pDE][3] - L.px[pCD][3],
-               xqD = L.px[pDE][4] - L.px[pCD][4];
+  const double q1B = rBC[2] - rAB[2], q2B = rBC[3] - rAB[3], xqB = rBC[4] - rAB[4];
+  const double q1D = rDE[2] - rCD[2], q2D = rDE[3] - rCD[3], xqD = rDE[4] - rCD[4];
   g.dB = q1B - g.nB * s;
   g.d2B = q2B - 2.0 * s * q1B + g.nB * s * s;
   g.xdB = xqB - s * g.sB;
   g.dD = q1D - g.nD * s;
   g.d2D = q2D - 2.0 * s * q1D + g.nD * s * s;
   g.xdD = xqD - s * g.sD;
-  return tf_eval(g, W, Lw, invL, Hout);
+  return tf_eval(g, W, Lw, invL, numB, denA);
 }
 
 static_assert(sizeof(TfLds) >= (kTfMaxBuckets + 1) * sizeof(int), "bucket counters exceed LDS");
@@ -684,7 +720,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
     }
 
     // ---- 2/3. epoch tiles ---------------------------------------------
-    TfBest best{-INFINITY, NAN, -1, -1, -1};
+    TfBest best{-INFINITY, NAN, NAN, -1, -1, -1};
     // first guess of the tile size from the mean phase density
     const double per_epoch = (double)nv / P * A.dtep;
     int te = (int)(((double)kTfChunk / fmax(per_epoch, 1e-9) - 2.0 * (hwm + marg) / A.dtep) - 3.0);
@@ -737,30 +773,34 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
               w -= c2 * nw;
               je += dje + c2;
             }
-            const int j = je * nw + w, ep = eb + je, m = mm + 1;
+            // row offsets by 24-bit multiplies (all operands < 2^24)
+            const unsigned j = __umul24((unsigned)je, (unsigned)nw) + (unsigned)w;
+            const int ep = eb + je, m = mm + 1;
             const double t0 = (double)ep * A.dtep, s = t0 - r, W = sw[w];
-            const int p0 = ewp[3 * j], pE = ewp[3 * j + 1];
-            const double *bd = wlt + kTfWl * (w * nl + m);
-            const double tAB = bd[0], tBC = bd[1], tCD = bd[2], tDE = bd[3];
-            int aAB, bAB, aBC, bBC, aCD, bCD, aDE, bDE;
-            tf_bracket(L, t0, tAB, aAB, bAB);
-            tf_bracket(L, t0, tBC, aBC, bBC);
-            tf_bracket(L, t0, tCD, aCD, bCD);
-            tf_bracket(L, t0, tDE, aDE, bDE);
-            int rAB = tf_count(L, aAB, bAB, t0, tAB), rBC = tf_count(L, aBC, bBC, t0, tBC),
-                rCD = tf_count(L, aCD, bCD, t0, tCD), rDE = tf_count(L, aDE, bDE, t0, tDE);
-            if (bAB - aAB > kTfCount) rAB = tf_search(L.lu, aAB, bAB, t0, tAB, 0);
-            if (bBC - aBC > kTfCount) rBC = tf_search(L.lu, aBC, bBC, t0, tBC, 0);
-            if (bCD - aCD > kTfCount) rCD = tf_search(L.lu, aCD, bCD, t0, tCD, 0);
-            if (bDE - aDE > kTfCount) rDE = tf_search(L.lu, aDE, bDE, t0, tDE, 0);
-            const int pAB = tf_clamp(rAB, p0, pE);
-            const int pBC = tf_clamp(rBC, pAB, pE);
-            const int pCD = tf_clamp(rCD, pBC, pE);
-            const int pDE = tf_clamp(rDE, pCD, pE);
-            double H;
-            const double sn = tf_stat_px(L, p0, pAB, pBC, pCD, pDE, pE, ewd[2 * j], ewd[2 * j + 1],
-                                         s, W, bd[4], bd[5], &H);
-            if (tf_better(sn, w, m, ep, best)) best = TfBest{sn, H, w, m, ep};
+            const int *wp = ewp + __umul24(j, 3u);
+            const double *wd = ewd + 2 * j;
+            const int p0 = wp[0], pE = wp[1];
+            const double *bd =
+                wlt + __umul24((unsigned)kTfWl, __umul24((unsigned)w, (unsigned)nl) + (unsigned)m);
+            const double cb[4] = {bd[0], bd[1], bd[2], bd[3]};
+            int ba[4], bb[4], br[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) tf_bracket(L, t0, cb[i], ba[i], bb[i]);
+            tf_count4(L, ba, bb, t0, cb, br);
+            if ((bb[0] - ba[0] > kTfCount) | (bb[1] - ba[1] > kTfCount) |
+                (bb[2] - ba[2] > kTfCount) | (bb[3] - ba[3] > kTfCount)) {
+#pragma unroll
+              for (int i = 0; i < 4; ++i)
+                if (bb[i] - ba[i] > kTfCount) br[i] = tf_search(L.lu, ba[i], bb[i], t0, cb[i], 0);
+            }
+            const int pAB = tf_clamp(br[0], p0, pE);
+            const int pBC = tf_clamp(br[1], pAB, pE);
+            const int pCD = tf_clamp(br[2], pBC, pE);
+            const int pDE = tf_clamp(br[3], pCD, pE);
+            double nB, dA;
+            const double key = tf_stat_px(L, p0, pAB, pBC, pCD, pDE, pE, wd[0], wd[1], s, W, bd[4],
+                                          bd[5], &nB, &dA);
+            if (tf_better(key, w, m, ep, best)) best = TfBest{key, nB, dA, w, m, ep};
           }
           __syncthreads();
         }
@@ -818,12 +858,12 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
             const double *v = base + 6 * b;
             return TfSums{v[0], v[1], v[2], v[3], v[4], v[5]};
           };
-          double H;
-          const double sn = tf_stat(get(0), get(3 + 4 * mm), get(4 + 4 * mm), get(5 + 4 * mm),
-                                    get(6 + 4 * mm), get(1), get(2), W,
-                                    wlt[kTfWl * (w * nl + m) + 4], wlt[kTfWl * (w * nl + m) + 5],
-                                    &H);
-          if (tf_better(sn, w, m, e0 + el, best)) best = TfBest{sn, H, w, m, e0 + el};
+          double nB, dA;
+          const double key = tf_stat(get(0), get(3 + 4 * mm), get(4 + 4 * mm), get(5 + 4 * mm),
+                                     get(6 + 4 * mm), get(1), get(2), W,
+                                     wlt[kTfWl * (w * nl + m) + 4], wlt[kTfWl * (w * nl + m) + 5],
+                                     &nB, &dA);
+          if (tf_better(key, w, m, e0 + el, best)) best = TfBest{key, nB, dA, w, m, e0 + el};
         }
         __syncthreads();
         e0 += tce;
@@ -835,26 +875,27 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
     }
     // ---- reduce the per-thread bests (wave shuffles, then LDS) ----
     for (int o = kWave / 2; o > 0; o >>= 1) {
-      const double osn = __shfl_xor(best.sn, o), oh = __shfl_xor(best.h, o);
+      const double ok = __shfl_xor(best.key, o), onb = __shfl_xor(best.numB, o),
+                   oda = __shfl_xor(best.denA, o);
       const int ow = __shfl_xor(best.w, o), ol = __shfl_xor(best.l, o),
                 oe = __shfl_xor(best.e, o);
-      if (ow >= 0 && (best.w < 0 || tf_better(osn, ow, ol, oe, best)))
-        best = TfBest{osn, oh, ow, ol, oe};
+      if (ow >= 0 && (best.w < 0 || tf_better(ok, ow, ol, oe, best)))
+        best = TfBest{ok, onb, oda, ow, ol, oe};
     }
     if ((tid & (kWave - 1)) == 0) {
       const int wv = tid / kWave;
-      L.red_sn[wv] = best.sn; L.red_h[wv] = best.h;
+      L.red_key[wv] = best.key; L.red_nb[wv] = best.numB; L.red_da[wv] = best.denA;
       L.red_w[wv] = best.w; L.red_l[wv] = best.l; L.red_e[wv] = best.e;
     }
     __syncthreads();
     if (tid == 0) {
-      TfBest b{L.red_sn[0], L.red_h[0], L.red_w[0], L.red_l[0], L.red_e[0]};
+      TfBest b{L.red_key[0], L.red_nb[0], L.red_da[0], L.red_w[0], L.red_l[0], L.red_e[0]};
       for (int k = 1; k < kTfThreads / kWave; ++k)
         if (L.red_w[k] >= 0 &&
-            (b.w < 0 || tf_better(L.red_sn[k], L.red_w[k], L.red_l[k], L.red_e[k], b)))
-          b = TfBest{L.red_sn[k], L.red_h[k], L.red_w[k], L.red_l[k], L.red_e[k]};
-      A.sn[item] = b.sn;
-      if (A.hbest) A.hbest[item] = b.h;
+            (b.w < 0 || tf_better(L.red_key[k], L.red_w[k], L.red_l[k], L.red_e[k], b)))
+          b = TfBest{L.red_key[k], L.red_nb[k], L.red_da[k], L.red_w[k], L.red_l[k], L.red_e[k]};
+      A.sn[item] = tf_sn(b.key);
+      if (A.hbest) A.hbest[item] = b.w >= 0 ? tf_h(b.numB, b.denA) : NAN;
       A.bw[item] = b.w;
       A.bl[item] = b.l;
       A.be[item] = b.e;
