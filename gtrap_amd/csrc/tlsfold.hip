@@ -160,27 +160,31 @@ __device__ __forceinline__ void tf_window(double mcnt, double sM, double nAll, d
 // sums, so res denA^2 = y2 denA^2 - numB denA U + numB^2 V, and
 // S/N = -H / sqrt(res/(n-3)) has the rank key
 // k = sign(numB denA) numB^2 (n-3) / (res denA^2): one division per trial.
-// invL = 1/L.  Writes numB and denA; NaN for no trial (n <= 3, denA = 0).
-__device__ __forceinline__ double tf_eval(const TfRegions &g, double W, double L, double invL,
+// c is the trial's (W, L) table row (products of L and W at 4..9).  Writes
+// numB and denA; NaN for no trial (n <= 3, denA = 0).  Multiply-adds may
+// fuse here: the statistic's rounding is not part of any region membership.
+__device__ __forceinline__ double tf_eval(const TfRegions &g, double W, const double *c,
                                           double *numBout, double *denAout) {
+#pragma clang fp contract(fast)
+  const double c4L2 = c[4], c8L2 = c[5], c4WL = c[6], c16L = c[7], c2iL = c[8], ciL2 = c[9];
+  const double q4 = 0.25 * W, q42 = q4 * q4, W2 = W * W, W8 = 8.0 * W;
   const double mean = g.mean;
   const double yA = g.sA - mean * g.nA, yB = g.sB - mean * g.nB, yC = g.sC - mean * g.nC,
                yD = g.sD - mean * g.nD, yE = g.sE - mean * g.nE;
   const double ydB = g.xdB - mean * g.dB, ydD = g.xdD - mean * g.dD;
-  const double xAEC = yA + yE + yC, xBD = yB + yD;
-  const double denA = 4.0 * L * L * (g.nA - g.nC + g.nE) + W * W * (g.nB + g.nD) +
-                      8.0 * W * (g.dB - g.dD) + 16.0 * (g.d2B + g.d2D);
-  const double numB = 8.0 * L * L * xAEC - 4.0 * W * L * xBD + 16.0 * L * (ydD - ydB);
-  const double q4 = W / 4.0;
+  const double nAE = g.nA + g.nE, nBD = g.nB + g.nD, yAE = yA + yE;
+  const double denA =
+      c4L2 * (nAE - g.nC) + W2 * nBD + W8 * (g.dB - g.dD) + 16.0 * (g.d2B + g.d2D);
+  const double numB = c8L2 * (yAE + yC) - c4WL * (yB + yD) + c16L * (ydD - ydB);
   // res = y2 + 2h(yA - yC + yE) + h^2(nA + nC + nE) - 2a(ydB + q4 yB)
   //       + a^2(d2B + 2 q4 dB + q4^2 nB) + 2a(ydD - q4 yD) + a^2(d2D - 2 q4 dD + q4^2 nD),
-  // h = H/2, a = H/L
-  const double U = (yA - yC + yE) + 2.0 * invL * ((ydD - q4 * yD) - (ydB + q4 * yB));
-  const double V = 0.25 * (g.nA + g.nC + g.nE) +
-                   invL * invL * ((g.d2B + 2.0 * q4 * g.dB + q4 * q4 * g.nB) +
-                                  (g.d2D - 2.0 * q4 * g.dD + q4 * q4 * g.nD));
+  // h = H/2, a = H/L, q4 = W/4
+  const double U = (yAE - yC) + c2iL * ((ydD - q4 * yD) - (ydB + q4 * yB));
+  const double V = 0.25 * (nAE + g.nC) +
+                   ciL2 * ((g.d2B + 2.0 * q4 * g.dB + q42 * g.nB) +
+                           (g.d2D - 2.0 * q4 * g.dD + q42 * g.nD));
   const double R = g.y2 * denA * denA - numB * denA * U + numB * numB * V;
-  const double nn = g.nA + g.nB + g.nC + g.nD + g.nE;
+  const double nn = nAE + nBD + g.nC;  // integers: exact in any order
   *numBout = numB;
   *denAout = denA;
   constexpr double kNaN = __builtin_nan("");
@@ -208,7 +212,7 @@ __device__ __forceinline__ double tf_h(double numB, double denA) { return -numB 
 __device__ __forceinline__ double tf_stat(const TfSums &c0, const TfSums &cAB,
                                           const TfSums &cBC, const TfSums &cCD,
                                           const TfSums &cDE, const TfSums &cE,
-                                          const TfSums &cM, double W, double L, double invL,
+                                          const TfSums &cM, double W, const double *c,
                                           double *numB, double *denA) {
   TfRegions g;
   tf_window(cM.n - c0.n, cM.s1 - c0.s1, cE.n - c0.n, cE.s1 - c0.s1, cE.s2 - c0.s2, g.mean, g.y2);
@@ -219,7 +223,7 @@ __device__ __forceinline__ double tf_stat(const TfSums &c0, const TfSums &cAB,
   g.dB = cBC.d1 - cAB.d1; g.dD = cDE.d1 - cCD.d1;
   g.d2B = cBC.d2 - cAB.d2; g.d2D = cDE.d2 - cCD.d2;
   g.xdB = cBC.xd - cAB.xd; g.xdD = cDE.xd - cCD.xd;
-  return tf_eval(g, W, L, invL, numB, denA);
+  return tf_eval(g, W, c, numB, denA);
 }
 
 struct TfArgs {
@@ -251,7 +255,8 @@ struct TfArgs {
 
 // LDS layout for a staged range
 constexpr int kTfMaxTe = 48;     // epochs per window-table run (at most)
-constexpr int kTfWl = 6;         // (W, L) table row: tAB, tBC, tCD, tDE, L, 1/L
+constexpr int kTfWl = 10;  // (W, L) table row: tAB, tBC, tCD, tDE, 4L^2, 8L^2, 4WL, 16L,
+                           // 2/L, 1/L^2
 constexpr int kTfCells = 2 * kTfChunk;  // phase index cells of a staged range (at most)
 constexpr int kTfMaxTile = 96;   // epochs per staged tile: with the reference at the
                                  // tile centre, |q| stays within 48 epoch steps + W/2
@@ -263,7 +268,7 @@ constexpr int kTfMaxBreaks = 64;      // sort chunks per period
 // Static LDS of a workgroup.  The per-(W, boundary) accumulators of the chunked
 // sweep and the per-(epoch, W) window table are sized at launch (dynamic LDS).
 struct TfLds {
-  double lu[kTfChunk];          // staged images (exact)
+  double lu[kTfChunk + 8];      // staged images (exact); 8 pad entries for tf_count4
   double px[kTfChunk + 1][5];   // exclusive prefix of x', x'^2, q, q^2, x'q
   // phase index of the staged range (two cells per sample): tab[g] = first k
   // with cell(lu[k]) >= g, cell(v) = clamp(floor((v - tbase) * tinv), 0, tnc - 1),
@@ -444,7 +449,9 @@ __device__ __forceinline__ int tf_find(const TfLds &L, int lo, int hi, double t0
 // brackets [a[i], b[i]) of at most kTfCount samples, by a branch-free
 // bisection of fixed depth (steps 4, 2, 1) with the four searches in step:
 // [a[i], r[i]) holds only failing samples throughout, and a step advances
-// r[i] past samples inside the bracket only.  Indices stay inside lu.
+// r[i] past samples inside the bracket only.  Reads reach at most index
+// a[i] + 6 <= kTfChunk + 6, inside lu's padding; values read past b[i] are
+// masked.
 constexpr int kTfCount = 7;
 __device__ __forceinline__ void tf_count4(const TfLds &L, const int *a, const int *b, double t0,
                                           const double *c, int *r) {
@@ -456,7 +463,7 @@ __device__ __forceinline__ void tf_count4(const TfLds &L, const int *a, const in
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int k = r[i] + st - 1;
-      d[i] = L.lu[min(k, kTfChunk - 1)] - t0;
+      d[i] = L.lu[k] - t0;
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -472,8 +479,9 @@ __device__ __forceinline__ void tf_count4(const TfLds &L, const int *a, const in
 // only for B and D), shifted to epoch coordinates (d = q - s) per region.
 __device__ __forceinline__ double tf_stat_px(const TfLds &L, int p0, int pAB, int pBC, int pCD,
                                              int pDE, int pE, double mean, double y2, double s,
-                                             double W, double Lw, double invL, double *numB,
+                                             double W, const double *c, double *numB,
                                              double *denA) {
+#pragma clang fp contract(fast)
   const double *r0 = tf_px(L, p0), *rAB = tf_px(L, pAB), *rBC = tf_px(L, pBC),
                *rCD = tf_px(L, pCD), *rDE = tf_px(L, pDE), *rE = tf_px(L, pE);
   const double x0 = r0[0], xAB = rAB[0], xBC = rBC[0], xCD = rCD[0], xDE = rDE[0], xE = rE[0];
@@ -491,7 +499,7 @@ __device__ __forceinline__ double tf_stat_px(const TfLds &L, int p0, int pAB, in
   g.dD = q1D - g.nD * s;
   g.d2D = q2D - 2.0 * s * q1D + g.nD * s * s;
   g.xdD = xqD - s * g.sD;
-  return tf_eval(g, W, Lw, invL, numB, denA);
+  return tf_eval(g, W, c, numB, denA);
 }
 
 static_assert(sizeof(TfLds) >= (kTfMaxBuckets + 1) * sizeof(int), "bucket counters exceed LDS");
@@ -513,7 +521,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
   // trial's S/N is NaN, which never wins, so only m = 1..nl-1 are evaluated
   const int nlt = nl - 1;
   const int nbnd = 3 + 4 * nlt;
-  // dynamic LDS: per-(W, L) region offsets and L, the widths, then the
+  // dynamic LDS: per-(W, L) region offsets and products of L, the widths, then the
   // per-(epoch, W) window table
   double *wlt = reinterpret_cast<double *>(tf_dyn);  // [nw*nl][kTfWl]
   double *sw = wlt + (size_t)kTfWl * nw * nl;        // [nw]
@@ -524,8 +532,13 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
     const double W = A.widths[w];
     double *row = wlt + kTfWl * k;
     tf_bounds(W, m, nl, row[0], row[1], row[2], row[3]);
-    row[4] = (double)m * (0.45 * W / (double)nl);
-    row[5] = 1.0 / row[4];
+    const double Lm = (double)m * (0.45 * W / (double)nl);  // L = 0 rows are never used
+    row[4] = 4.0 * Lm * Lm;
+    row[5] = 8.0 * Lm * Lm;
+    row[6] = 4.0 * W * Lm;
+    row[7] = 16.0 * Lm;
+    row[8] = 2.0 / Lm;
+    row[9] = 1.0 / (Lm * Lm);
   }
   for (int w = tid; w < nw; w += kTfThreads) sw[w] = A.widths[w];
   __syncthreads();
@@ -798,8 +811,8 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
             const int pCD = tf_clamp(br[2], pBC, pE);
             const int pDE = tf_clamp(br[3], pCD, pE);
             double nB, dA;
-            const double key = tf_stat_px(L, p0, pAB, pBC, pCD, pDE, pE, wd[0], wd[1], s, W, bd[4],
-                                          bd[5], &nB, &dA);
+            const double key =
+                tf_stat_px(L, p0, pAB, pBC, pCD, pDE, pE, wd[0], wd[1], s, W, bd, &nB, &dA);
             if (tf_better(key, w, m, ep, best)) best = TfBest{key, nB, dA, w, m, ep};
           }
           __syncthreads();
@@ -861,8 +874,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
           double nB, dA;
           const double key = tf_stat(get(0), get(3 + 4 * mm), get(4 + 4 * mm), get(5 + 4 * mm),
                                      get(6 + 4 * mm), get(1), get(2), W,
-                                     wlt[kTfWl * (w * nl + m) + 4], wlt[kTfWl * (w * nl + m) + 5],
-                                     &nB, &dA);
+                                     wlt + kTfWl * (w * nl + m), &nB, &dA);
           if (tf_better(key, w, m, e0 + el, best)) best = TfBest{key, nB, dA, w, m, e0 + el};
         }
         __syncthreads();
