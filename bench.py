@@ -22,9 +22,10 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK = 78.6  # TFLOP/s, MI355X dense FP64
 # FP64 operations of one (epoch, W, L) trial from the tile prefix sums: the
 # region sums from prefix differences, shifted to epoch coordinates for the
-# ingress and egress regions (34), and the closed-form trapezoid fit,
-# residual and S/N (101); +, -, *, /, sqrt count one each
-TRIAL_FLOPS = 135
+# ingress and egress regions (29), and the closed-form trapezoid fit,
+# residual and rank key sign(S/N)*(S/N)^2 (74); +, -, *, / count one each,
+# a fused multiply-add two
+TRIAL_FLOPS = 103
 # HBM bytes per serving-step fold launch of the default serve workload,
 # measured offline: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch of
 # tlsfold_kernel<true>, scaled by the same counters over a 1 GiB copy

@@ -37,9 +37,11 @@ constexpr int kTfMaxBnd = 3 + 4 * kTfMaxNl;  // boundaries per W
 
 // fmod(t, P) for t >= 0, P > 0, exactly: the remainder t - q*P of the
 // integer quotient q is representable, so one fused multiply-add returns it
-// exactly; q from the rounded division is corrected by one step when needed.
-__device__ __forceinline__ double tf_fmod(double t, double P) {
-  double q = floor(t / P);
+// exactly.  q = floor(t * invP) (invP = 1/P rounded) is within one of
+// floor(t/P) (absolute error below 2^-2 for quotients below 2^50), and one
+// correction step makes it exact.
+__device__ __forceinline__ double tf_fmod(double t, double P, double invP) {
+  double q = floor(t * invP);
   double r = fma(-q, P, t);
   if (r < 0.0) { q -= 1.0; r = fma(-q, P, t); }
   else if (r >= P) { q += 1.0; r = fma(-q, P, t); }
@@ -264,6 +266,7 @@ constexpr int kTfMaxT = 4;    // chunked sweep: (epoch, W, boundary) sums per th
 constexpr int kTfMaxBuckets = 30000;  // phase buckets (LDS counters, <= 120 KB)
 constexpr int kTfSortChunk = 7680;    // samples per in-LDS bucket-sort chunk (90 KB + ends)
 constexpr int kTfMaxBreaks = 64;      // sort chunks per period
+constexpr int kTfUnroll = 4;          // global loads in flight per thread in the sort loops
 
 // Static LDS of a workgroup.  The per-(W, boundary) accumulators of the chunked
 // sweep and the per-(epoch, W) window table are sized at launch (dynamic LDS).
@@ -309,6 +312,27 @@ constexpr int kPer = kTfChunk / kTfThreads;
 __device__ void tf_stage(TfLds &L, const double *su, const float *sx, int nv, double P, int lo,
                          int cnt, double r, double xref) {
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+  // coalesced loads (element u * kTfThreads + tid, every load in flight before
+  // the first use) into lu and, as x' - xref, into the rows' first prefix slot
+  if (cnt > 0) {
+    double iv[kPer];
+    float xv[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int js = tf_src(nv, lo + min(u * kTfThreads + tid, cnt - 1));
+      iv[u] = su[js];
+      xv[u] = sx[js];
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int k = u * kTfThreads + tid, j = lo + k;
+      if (k < cnt) {
+        L.lu[k] = j < 0 ? iv[u] - P : (j >= nv ? iv[u] + P : iv[u]);  // tf_img
+        L.px[k][0] = (double)xv[u] - xref;
+      }
+    }
+  }
+  __syncthreads();
   double xs[kPer], qs[kPer];
   double acc[5] = {0, 0, 0, 0, 0};
 #pragma unroll
@@ -317,10 +341,8 @@ __device__ void tf_stage(TfLds &L, const double *su, const float *sx, int nv, do
     xs[u] = 0.0;
     qs[u] = 0.0;
     if (k < cnt) {
-      const int j = lo + k;
-      const double img = tf_img(su, nv, P, j);
-      L.lu[k] = img;
-      xs[u] = (double)sx[tf_src(nv, j)] - xref;
+      const double img = L.lu[k];
+      xs[u] = L.px[k][0];
       qs[u] = img - r;
       acc[0] += xs[u]; acc[1] += xs[u] * xs[u]; acc[2] += qs[u]; acc[3] += qs[u] * qs[u];
       acc[4] += xs[u] * qs[u];
@@ -474,6 +496,21 @@ __device__ __forceinline__ void tf_count4(const TfLds &L, const int *a, const in
   }
 }
 
+// One boundary of either kind (0: d >= c, 1: d > c) in a bracket of at most
+// kTfCount samples, by the same fixed-depth bisection as tf_count4.
+__device__ __forceinline__ int tf_bisect(const TfLds &L, int a, int b, double t0, double c,
+                                         int kind) {
+  int r = a;
+#pragma unroll
+  for (int st = 4; st > 0; st >>= 1) {
+    const int k = r + st - 1;
+    const double d = L.lu[k] - t0;
+    const int fail = (int)(k < b) & (int)!(kind ? d > c : d >= c);
+    r += fail * st;
+  }
+  return r;
+}
+
 // The statistic from the staged prefix sums at the trial's boundary indices:
 // raw differences first (x', x'^2 for the window and regions; q, q^2, x'q
 // only for B and D), shifted to epoch coordinates (d = q - s) per region.
@@ -542,6 +579,8 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
   }
   for (int w = tid; w < nw; w += kTfThreads) sw[w] = A.widths[w];
   __syncthreads();
+  // this thread's first (epoch, W) entry of a window-table run
+  const int wt_e = tid / nw, wt_w = tid % nw;
   // advance of the trial loop's (L, W, epoch) indices per kTfThreads trials
   const int dm = nlt ? kTfThreads % nlt : 0, dw = nlt ? (kTfThreads / nlt) % nw : 0,
             dje = nlt ? kTfThreads / (nlt * nw) : 0;
@@ -592,18 +631,26 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
     // the number of valid samples and the first valid index)
     const int nb = max(1, min(n / 2, kTfMaxBuckets));
     const double bscale = (double)nb / P;
+    const double invP = 1.0 / P;
     int *lcur = reinterpret_cast<int *>(&L);  // nb + 1 <= kTfMaxBuckets + 1 ints
     if (tid == 0) { s_cnt = 0; s_i0 = n; }
     for (int b = tid; b <= nb; b += kTfThreads) lcur[b] = 0;
     __syncthreads();
     {
       int myv = 0, myi0 = n;
-      for (int i = tid; i < n; i += kTfThreads) {
-        if (!(tc[i] >= 0.0f)) continue;
-        ++myv;
-        myi0 = min(myi0, i);
-        const double u = tf_fmod((double)tc[i], P);
-        atomicAdd(&lcur[min(nb - 1, (int)(u * bscale))], 1);
+      for (int i0 = tid; i0 < n; i0 += kTfUnroll * kTfThreads) {
+        float tv[kTfUnroll];  // every load in flight before the first use
+#pragma unroll
+        for (int v = 0; v < kTfUnroll; ++v) tv[v] = tc[min(i0 + v * kTfThreads, n - 1)];
+#pragma unroll
+        for (int v = 0; v < kTfUnroll; ++v) {
+          const int i = i0 + v * kTfThreads;
+          if (i >= n || !(tv[v] >= 0.0f)) continue;
+          ++myv;
+          myi0 = min(myi0, i);
+          const double u = tf_fmod((double)tv[v], P, invP);
+          atomicAdd(&lcur[min(nb - 1, (int)(u * bscale))], 1);
+        }
       }
       atomicAdd(&s_cnt, myv);
       atomicMin(&s_i0, myi0);
@@ -632,12 +679,22 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
       __syncthreads();
     }
     // scatter; afterwards lcur[b] is the end of bucket b
-    for (int i = tid; i < n; i += kTfThreads) {
-      if (!(tc[i] >= 0.0f)) continue;
-      const double u = tf_fmod((double)tc[i], P);
-      const int pos = atomicAdd(&lcur[min(nb - 1, (int)(u * bscale))], 1);
-      su[pos] = u;
-      sx[pos] = xc[i];
+    for (int i0 = tid; i0 < n; i0 += kTfUnroll * kTfThreads) {
+      float tv[kTfUnroll], xv[kTfUnroll];
+#pragma unroll
+      for (int v = 0; v < kTfUnroll; ++v) {
+        const int i = min(i0 + v * kTfThreads, n - 1);
+        tv[v] = tc[i];
+        xv[v] = xc[i];
+      }
+#pragma unroll
+      for (int v = 0; v < kTfUnroll; ++v) {
+        if (i0 + v * kTfThreads >= n || !(tv[v] >= 0.0f)) continue;
+        const double u = tf_fmod((double)tv[v], P, invP);
+        const int pos = atomicAdd(&lcur[min(nb - 1, (int)(u * bscale))], 1);
+        su[pos] = u;
+        sx[pos] = xv[v];
+      }
     }
     __syncthreads();
     // bucket-aligned sort chunks of at most kTfSortChunk samples (greedy, from
@@ -670,7 +727,21 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
         const int s1 = cursor[be - 1];
         const int cnt = s1 - s0;
         if (cnt <= kTfSortChunk && be - bb <= kTfSortChunk) {
-          for (int k = tid; k < cnt; k += kTfThreads) { cu[k] = su[s0 + k]; cx[k] = sx[s0 + k]; }
+          for (int k0 = tid; k0 < cnt; k0 += kTfUnroll * kTfThreads) {
+            double uv[kTfUnroll];
+            float xv[kTfUnroll];
+#pragma unroll
+            for (int v = 0; v < kTfUnroll; ++v) {
+              const int k = min(k0 + v * kTfThreads, cnt - 1);
+              uv[v] = su[s0 + k];
+              xv[v] = sx[s0 + k];
+            }
+#pragma unroll
+            for (int v = 0; v < kTfUnroll; ++v) {
+              const int k = k0 + v * kTfThreads;
+              if (k < cnt) { cu[k] = uv[v]; cx[k] = xv[v]; }
+            }
+          }
           for (int k = tid; k < be - bb; k += kTfThreads) cb[k] = cursor[bb + k] - s0;
           __syncthreads();
           // rank of every sample inside its bucket under the total order
@@ -760,11 +831,25 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
           const int nte = min(A.te_run, e1 - eb);
           // (epoch, W) window table: p0 (d >= -W/2), pE (d >= W/2), pM (d > W/2)
           for (int k = tid; k < nte * nw; k += kTfThreads) {
-            const int ep = eb + k / nw, w = k % nw;
+            const int ep = eb + (k == tid ? wt_e : k / nw), w = k == tid ? wt_w : k % nw;
             const double t0 = (double)ep * A.dtep, W = sw[w];
-            const int p0 = tf_find(L, 0, cnt, t0, -0.5 * W, 0);
-            const int pE = tf_find(L, p0, cnt, t0, 0.5 * W, 0);
-            const int pM = tf_find(L, pE, cnt, t0, 0.5 * W, 1);
+            // p0: d >= -W/2; pE: d >= W/2 and pM: d > W/2 share one bracket
+            int a0, b0, a1, b1;
+            tf_bracket(L, t0, -0.5 * W, a0, b0);
+            tf_bracket(L, t0, 0.5 * W, a1, b1);
+            int p0 = tf_bisect(L, a0, b0, t0, -0.5 * W, 0);
+            int pE = tf_bisect(L, a1, b1, t0, 0.5 * W, 0);
+            int pM = tf_bisect(L, a1, b1, t0, 0.5 * W, 1);
+            if ((b0 - a0 > kTfCount) | (b1 - a1 > kTfCount)) {
+              if (b0 - a0 > kTfCount) p0 = tf_search(L.lu, a0, b0, t0, -0.5 * W, 0);
+              if (b1 - a1 > kTfCount) {
+                pE = tf_search(L.lu, a1, b1, t0, 0.5 * W, 0);
+                pM = tf_search(L.lu, a1, b1, t0, 0.5 * W, 1);
+              }
+            }
+            p0 = tf_clamp(p0, 0, cnt);
+            pE = tf_clamp(pE, p0, cnt);
+            pM = tf_clamp(pM, pE, cnt);
             ewp[3 * k] = p0; ewp[3 * k + 1] = pE; ewp[3 * k + 2] = pM;
             tf_window((double)(pM - p0), L.px[pM][0] - L.px[p0][0], (double)(pE - p0),
                       L.px[pE][0] - L.px[p0][0], L.px[pE][1] - L.px[p0][1], ewd[2 * k],
