@@ -84,13 +84,16 @@ __device__ __forceinline__ void tf_bounds(double W, int m, int nl, double &tAB, 
   tDE = qw + hl;
 }
 
-// Image of extended index j of the sorted folded curve (nv samples):
-// j < 0 -> u[j+nv] - P, j >= nv -> u[j-nv] + P.
-__device__ __forceinline__ double tf_img(const double *su, int nv, double P, int j) {
-  if (j < 0) return su[j + nv] - P;
-  if (j >= nv) return su[j - nv] + P;
-  return su[j];
-}
+// One sample of the phase-sorted curve in the workspace: phase u, flux x.
+// Sixteen bytes, so a sample moves in one store or load.
+struct __align__(16) TfRec {
+  double u;
+  float x;
+  int pad;
+};
+
+// Extended index j of the sorted folded curve (nv samples) maps to sample
+// tf_src(nv, j); its image is u - P for j < 0, u + P for j >= nv, else u.
 __device__ __forceinline__ int tf_src(int nv, int j) {
   return j < 0 ? j + nv : (j >= nv ? j - nv : j);
 }
@@ -309,7 +312,7 @@ __device__ __forceinline__ int tf_cell(double v, double base, double inv, int nc
 }
 
 constexpr int kPer = kTfChunk / kTfThreads;
-__device__ void tf_stage(TfLds &L, const double *su, const float *sx, int nv, double P, int lo,
+__device__ void tf_stage(TfLds &L, const TfRec *rec, int nv, double P, int lo,
                          int cnt, double r, double xref) {
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
   // coalesced loads (element u * kTfThreads + tid, every load in flight before
@@ -320,14 +323,15 @@ __device__ void tf_stage(TfLds &L, const double *su, const float *sx, int nv, do
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
       const int js = tf_src(nv, lo + min(u * kTfThreads + tid, cnt - 1));
-      iv[u] = su[js];
-      xv[u] = sx[js];
+      const TfRec rv = rec[js];
+      iv[u] = rv.u;
+      xv[u] = rv.x;
     }
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
       const int k = u * kTfThreads + tid, j = lo + k;
       if (k < cnt) {
-        L.lu[k] = j < 0 ? iv[u] - P : (j >= nv ? iv[u] + P : iv[u]);  // tf_img
+        L.lu[k] = j < 0 ? iv[u] - P : (j >= nv ? iv[u] + P : iv[u]);
         L.px[k][0] = (double)xv[u] - xref;
       }
     }
@@ -585,9 +589,8 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
   const int dm = nlt ? kTfThreads % nlt : 0, dw = nlt ? (kTfThreads / nlt) % nw : 0,
             dje = nlt ? kTfThreads / (nlt * nw) : 0;
   unsigned char *ws = A.ws + 256 + (size_t)blockIdx.x * (size_t)A.ws_per_block;
-  double *su = reinterpret_cast<double *>(ws);
-  float *sx = reinterpret_cast<float *>(su + A.nmax);
-  int *cursor = reinterpret_cast<int *>(sx + A.nmax + (A.nmax & 1));
+  TfRec *rec = reinterpret_cast<TfRec *>(ws);
+  int *cursor = reinterpret_cast<int *>(rec + A.nmax);
   int *ticket = A.counter ? A.counter : reinterpret_cast<int *>(A.ws);
   double wmax = 0.0;
   for (int w = 0; w < nw; ++w) wmax = fmax(wmax, A.widths[w]);
@@ -692,8 +695,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
         if (i0 + v * kTfThreads >= n || !(tv[v] >= 0.0f)) continue;
         const double u = tf_fmod((double)tv[v], P, invP);
         const int pos = atomicAdd(&lcur[min(nb - 1, (int)(u * bscale))], 1);
-        su[pos] = u;
-        sx[pos] = xv[v];
+        rec[pos] = TfRec{u, xv[v], 0};
       }
     }
     __syncthreads();
@@ -733,8 +735,9 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
 #pragma unroll
             for (int v = 0; v < kTfUnroll; ++v) {
               const int k = min(k0 + v * kTfThreads, cnt - 1);
-              uv[v] = su[s0 + k];
-              xv[v] = sx[s0 + k];
+              const TfRec rv = rec[s0 + k];
+              uv[v] = rv.u;
+              xv[v] = rv.x;
             }
 #pragma unroll
             for (int v = 0; v < kTfUnroll; ++v) {
@@ -760,8 +763,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
               rank += (ju < ku || (ju == ku && (jk < kk || (jk == kk && j < k)))) ? 1 : 0;
             }
             const int dst = s0 + min(a1 + rank, cnt - 1);
-            su[dst] = ku;
-            sx[dst] = kx;
+            rec[dst] = TfRec{ku, kx, 0};
           }
           __syncthreads();
         } else {
@@ -769,16 +771,14 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
           for (int b = bb + tid; b < be; b += kTfThreads) {
             const int a1 = b == 0 ? 0 : cursor[b - 1], a2 = cursor[b];
             for (int k = a1 + 1; k < a2; ++k) {
-              const double ku = su[k];
-              const float kx = sx[k];
+              const TfRec kr = rec[k];
               int j = k - 1;
-              while (j >= a1 && (su[j] > ku || (su[j] == ku && tf_xkey(sx[j]) > tf_xkey(kx)))) {
-                su[j + 1] = su[j];
-                sx[j + 1] = sx[j];
+              while (j >= a1 && (rec[j].u > kr.u ||
+                                 (rec[j].u == kr.u && tf_xkey(rec[j].x) > tf_xkey(kr.x)))) {
+                rec[j + 1] = rec[j];
                 --j;
               }
-              su[j + 1] = ku;
-              sx[j + 1] = kx;
+              rec[j + 1] = kr;
             }
           }
           __syncthreads();
@@ -788,16 +788,14 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
       for (int b = tid; b < nb; b += kTfThreads) {
         const int a1 = b == 0 ? 0 : cursor[b - 1], a2 = cursor[b];
         for (int k = a1 + 1; k < a2; ++k) {
-          const double ku = su[k];
-          const float kx = sx[k];
+          const TfRec kr = rec[k];
           int j = k - 1;
-          while (j >= a1 && (su[j] > ku || (su[j] == ku && tf_xkey(sx[j]) > tf_xkey(kx)))) {
-            su[j + 1] = su[j];
-            sx[j + 1] = sx[j];
+          while (j >= a1 && (rec[j].u > kr.u ||
+                             (rec[j].u == kr.u && tf_xkey(rec[j].x) > tf_xkey(kr.x)))) {
+            rec[j + 1] = rec[j];
             --j;
           }
-          su[j + 1] = ku;
-          sx[j + 1] = kx;
+          rec[j + 1] = kr;
         }
       }
       __syncthreads();
@@ -825,7 +823,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
       }
       if (cnt <= kTfChunk) {
         const double r = (double)((e0 + e1 - 1) / 2) * A.dtep;  // q reference: central epoch
-        tf_stage(L, su, sx, nv, P, lo, cnt, r, xref);
+        tf_stage(L, rec, nv, P, lo, cnt, r, xref);
         // the staged tile is swept in runs of te_run epochs
         for (int eb = e0; eb < e1; eb += A.te_run) {
           const int nte = min(A.te_run, e1 - eb);
@@ -919,7 +917,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
           for (int v = 0; v < 6; ++v) acc[j][v] = 0.0;
         for (int ca = lo; ca < chi; ca += kTfChunk) {
           const int cc = min(kTfChunk, chi - ca);
-          tf_stage(L, su, sx, nv, P, ca, cc, r, xref);
+          tf_stage(L, rec, nv, P, ca, cc, r, xref);
 #pragma unroll
           for (int j = 0; j < kTfMaxT; ++j) {
             const int k = tid + j * kTfThreads;
@@ -1034,10 +1032,10 @@ static int tf_launch_cfg(int nw, int nl, size_t *dyn, int *te_run) {
 }  // namespace gtrap
 
 // Workspace: a 256-byte header (work ticket) plus one slot per workgroup with
-// the phase-sorted curve (float64 phase, float32 flux) and the bucket offsets.
+// the phase-sorted curve (16-byte records: float64 phase, float32 flux) and the bucket offsets.
 extern "C" int64_t gtrap_tlsfold_ws_bytes(int32_t nmax, int32_t nblocks) {
   if (nmax < 1) nmax = 1;
-  int64_t per = (int64_t)nmax * 8 + (int64_t)(nmax + 1) * 4 + (int64_t)(nmax / 2 + 2) * 4;
+  int64_t per = (int64_t)nmax * 16 + (int64_t)(nmax / 2 + 2) * 4;
   per = (per + 255) / 256 * 256;  // keep every block's slot 256-byte aligned
   return 256 + per * (int64_t)(nblocks > 0 ? nblocks : 1);
 }
