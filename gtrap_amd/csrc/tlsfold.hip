@@ -464,13 +464,6 @@ __device__ __forceinline__ int tf_clamp(int r, int lo, int hi) {
   return r < lo ? lo : (r > hi ? hi : r);
 }
 
-__device__ __forceinline__ int tf_find(const TfLds &L, int lo, int hi, double t0, double c,
-                                       int kind) {
-  int a, b;
-  tf_bracket(L, t0, c, a, b);
-  return tf_clamp(tf_search(L.lu, a, b, t0, c, kind), lo, hi);
-}
-
 // First index passing the offset test (kind 0, d >= c) in each of four
 // brackets [a[i], b[i]) of at most kTfCount samples, by a branch-free
 // bisection of fixed depth (steps 4, 2, 1) with the four searches in step:
@@ -911,10 +904,24 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
                                         (double)(e0 + tce - 1) * A.dtep + hwm + marg, 1);
         const int ntrip = tce * nw * nbnd;
         double acc[kTfMaxT][6];
+        // each slot's (epoch, boundary), the same for every chunk of the pass
+        double jt0[kTfMaxT], jc[kTfMaxT];
+        int jkind[kTfMaxT];
 #pragma unroll
-        for (int j = 0; j < kTfMaxT; ++j)
+        for (int j = 0; j < kTfMaxT; ++j) {
 #pragma unroll
           for (int v = 0; v < 6; ++v) acc[j][v] = 0.0;
+          const int k = tid + j * kTfThreads;
+          jt0[j] = 0.0;
+          jc[j] = 0.0;
+          jkind[j] = 0;
+          if (k < ntrip) {
+            const int el = k / (nw * nbnd), rem = k % (nw * nbnd);
+            const int w = rem / nbnd, b = rem % nbnd;
+            jt0[j] = (double)(e0 + el) * A.dtep;
+            tf_bnd(sw[w], wlt + kTfWl * w * nl, b, jc[j], jkind[j]);
+          }
+        }
         for (int ca = lo; ca < chi; ca += kTfChunk) {
           const int cc = min(kTfChunk, chi - ca);
           tf_stage(L, rec, nv, P, ca, cc, r, xref);
@@ -922,13 +929,12 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
           for (int j = 0; j < kTfMaxT; ++j) {
             const int k = tid + j * kTfThreads;
             if (k < ntrip) {
-              const int el = k / (nw * nbnd), rem = k % (nw * nbnd);
-              const int w = rem / nbnd, b = rem % nbnd;
-              const double t0 = (double)(e0 + el) * A.dtep;
-              double cval;
-              int kind;
-              tf_bnd(sw[w], wlt + kTfWl * w * nl, b, cval, kind);
-              const int p = tf_find(L, 0, cc, t0, cval, kind);
+              const double t0 = jt0[j], cval = jc[j];
+              int a, b;
+              tf_bracket(L, t0, cval, a, b);
+              int p = tf_bisect(L, a, b, t0, cval, jkind[j]);
+              if (b - a > kTfCount) p = tf_search(L.lu, a, b, t0, cval, jkind[j]);
+              p = tf_clamp(p, 0, cc);
               const TfSums cs = tf_cum(L, p, t0 - r);
               acc[j][0] += cs.n; acc[j][1] += cs.s1; acc[j][2] += cs.s2;
               acc[j][3] += cs.d1; acc[j][4] += cs.d2; acc[j][5] += cs.xd;
