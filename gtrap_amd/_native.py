@@ -25,6 +25,11 @@ SIGNATURES: dict[str, tuple] = {
                             _u32, _u32, _f32, _f32, _u32, _vp]),
     "gtrap_geebls_simple": (_int, [_vp, _vp, _vp, _u32, _u32, _u32, _u32, _u32, _u32, _u32,
                                    _f32, _f32, _u32, _vp]),
+    "gtrap_geebls_ws_bytes": (_i64, [_u32, _u32, _i32]),
+    "gtrap_geebls_ws": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _u32, _u32, _u32, _u32,
+                               _u32, _u32, _f32, _f32, _u32, _vp, _i64, _vp]),
+    "gtrap_geebls_simple_ws": (_int, [_vp, _vp, _vp, _u32, _u32, _u32, _u32, _u32, _u32, _u32,
+                                      _f32, _f32, _u32, _vp, _i64, _vp]),
     "gtrap_geebls_prec": (_int, [_int] + [_vp] * 10 + [_u32] * 7 + [_f64, _f64, _u32, _vp]),
     "gtrap_image_build": (_int, [_vp] * 11 + [_i32, _i32, _i32, _f32, _f32, _int, _vp, _vp]),
     "gtrap_gamma": (_int, [_vp, _i64, _f64, _f64, C.c_uint64, _vp, _vp]),

@@ -17,6 +17,11 @@
 // float atomics do).  The scan keeps the reference's sequential float32
 // running sum per start bin (one LDS read per trial), and the argmax is a
 // deterministic wave/LDS reduction with a lowest-(start, end) tie-break.
+//
+// With a caller workspace (gtrap_geebls_ws) one pass first copies the
+// interleaved image to curve-major order and takes every curve's scale
+// (max |lc - offset|) once: the binning loads are then coalesced and no
+// workgroup gathers its whole curve for the scale.  Same results.
 #include "common.h"
 #include <math.h>
 
@@ -43,13 +48,57 @@ __device__ __forceinline__ bool better(float p, uint32_t i, uint32_t j, const Be
   return (p > b.p) || (p == b.p && (i < b.i || (i == b.i && j < b.j)));
 }
 
-template <bool SIMPLE>
+constexpr int kBtTile = 64;  // transpose tile: 64 samples x 64 curves
+
+__global__ void bls_zero_kernel(uint32_t *p, uint32_t count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = 0u;
+}
+
+// Interleaved (n, nq) lc (and tu unless null) to curve-major copies
+// (element (i, q) to q*n + i) through a padded LDS tile, and each curve's
+// max |lc - offset| over its n samples as float bits in maxbits[q] (fmaxf
+// from 0: NaN samples are skipped, as in the per-workgroup scale).
+__global__ __launch_bounds__(256) void bls_transpose_kernel(
+    const float *__restrict__ lc, const float *__restrict__ tu, const float *__restrict__ offset,
+    float *__restrict__ lct, float *__restrict__ tut, uint32_t *__restrict__ maxbits, uint32_t n,
+    uint32_t nq) {
+  __shared__ float sl[kBtTile][kBtTile + 1], st[kBtTile][kBtTile + 1];
+  const uint32_t q0 = blockIdx.x * kBtTile, i0 = blockIdx.y * kBtTile;
+  const int tx = threadIdx.x & (kBtTile - 1), ty = threadIdx.x / kBtTile;
+  const uint32_t q = q0 + tx;
+  const float off = (offset && q < nq) ? offset[q] : 0.0f;
+  float mx = 0.0f;
+  for (int r = ty; r < kBtTile; r += 256 / kBtTile) {
+    const uint32_t i = i0 + r;
+    float v = 0.0f, t = 0.0f;
+    if (i < n && q < nq) {
+      v = lc[(size_t)i * nq + q];
+      mx = fmaxf(mx, fabsf(v - off));
+      if (tu) t = tu[(size_t)i * nq + q];
+    }
+    sl[r][tx] = v;
+    st[r][tx] = t;
+  }
+  if (q < nq) atomicMax(&maxbits[q], __float_as_uint(mx));
+  __syncthreads();
+  for (int r = ty; r < kBtTile; r += 256 / kBtTile) {
+    const uint32_t qq = q0 + r, i = i0 + tx;
+    if (qq < nq && i < n) {
+      lct[(size_t)qq * n + i] = sl[tx][r];
+      if (tu) tut[(size_t)qq * n + i] = st[tx][r];
+    }
+  }
+}
+
+// T: curve-major lc/tu (tu 1-D when SIMPLE) and the scales in maxbits
+template <bool SIMPLE, bool T>
 __global__ __launch_bounds__(kBlsThreads) void bls_kernel(
     float *__restrict__ blsp, uint32_t *__restrict__ phase1, uint32_t *__restrict__ phase2,
     const float *__restrict__ lc, const float *__restrict__ tu, const float *__restrict__ offset,
-    const uint32_t *__restrict__ ntrue, uint32_t n, uint32_t nq, uint32_t nf, uint32_t nb,
-    uint32_t kma, uint32_t kmi, uint32_t kkmi, float fmin, float df, uint32_t nused,
-    int fpb) {
+    const uint32_t *__restrict__ ntrue, const uint32_t *__restrict__ maxbits, uint32_t n,
+    uint32_t nq, uint32_t nf, uint32_t nb, uint32_t kma, uint32_t kmi, uint32_t kkmi, float fmin,
+    float df, uint32_t nused, int fpb) {
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t nbk = nb + kma;
   // LDS: Y[fpb][nbk] int64 | C[fpb][nbk] u32 | yf[nbk] float (scan view)
@@ -71,17 +120,25 @@ __global__ __launch_bounds__(kBlsThreads) void bls_kernel(
   const float rnb = (float)nb;
 
   // per-curve fixed-point scale from max |lc - offset| over all n samples
+  // sample i of curve q: curve-major or interleaved
+  const auto at = [&](uint32_t i) -> size_t { return T ? (size_t)q * n + i : (size_t)i * nq + q; };
   float mx = 0.0f;
-  for (uint32_t i = tid; i < n; i += kBlsThreads) mx = fmaxf(mx, fabsf(lc[(size_t)i * nq + q] - off));
-  for (int o = kWave / 2; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
-  if ((tid & (kWave - 1)) == 0) red_max[tid / kWave] = mx;
+  if (!T) {
+    for (uint32_t i = tid; i < n; i += kBlsThreads) mx = fmaxf(mx, fabsf(lc[at(i)] - off));
+    for (int o = kWave / 2; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    if ((tid & (kWave - 1)) == 0) red_max[tid / kWave] = mx;
+  }
   for (uint32_t e = tid; e < (uint32_t)fpb * nbk; e += kBlsThreads) {
     Y[e] = 0;
     Cn[e] = 0;
   }
   __syncthreads();
-  mx = red_max[0];
-  for (int w = 1; w < kBlsThreads / kWave; ++w) mx = fmaxf(mx, red_max[w]);
+  if (T) {
+    mx = __uint_as_float(maxbits[q]);
+  } else {
+    mx = red_max[0];
+    for (int w = 1; w < kBlsThreads / kWave; ++w) mx = fmaxf(mx, red_max[w]);
+  }
   const int S = bls_scale_exp(mx, n);
 
   float f0s[8];
@@ -98,8 +155,8 @@ __global__ __launch_bounds__(kBlsThreads) void bls_kernel(
       float tx = -1.0f;
       int64_t v = 0;
       if (i < nused) {
-        tx = SIMPLE ? tu[i] : tu[(size_t)i * nq + q];
-        v = llrint(ldexp((double)(lc[(size_t)i * nq + q] - off), S));
+        tx = SIMPLE ? tu[i] : tu[at(i)];
+        v = llrint(ldexp((double)(lc[at(i)] - off), S));
       }
       tt[s] = tx;
       qv[s] = v;
@@ -187,12 +244,21 @@ __global__ __launch_bounds__(kBlsThreads) void bls_kernel(
   }
 }
 
+// Workspace of the curve-major path: per-curve scale bits, then lc and (unless
+// simple) tu in curve-major order, each 256-byte aligned.
+static size_t bls_ws_part(size_t bytes) { return (bytes + 255) / 256 * 256; }
+static size_t bls_ws_need(uint32_t n, uint32_t nq, bool simple) {
+  const size_t img = bls_ws_part((size_t)n * nq * sizeof(float));
+  return bls_ws_part((size_t)nq * sizeof(uint32_t)) + img * (simple ? 1 : 2);
+}
+
 static int bls_launch(bool simple, float *blsp, uint32_t *phase1, uint32_t *phase2,
                       const float *lc, const float *tu, const float *offset,
                       const uint32_t *ntrue, uint32_t n, uint32_t nq, uint32_t nf, uint32_t nb,
                       uint32_t kma, uint32_t kmi, uint32_t kkmi, float fmin, float df,
-                      uint32_t nthread, hipStream_t st) {
-  const char *fn = simple ? "gtrap_geebls_simple" : "gtrap_geebls";
+                      uint32_t nthread, void *ws, int64_t ws_bytes, hipStream_t st) {
+  const char *fn = simple ? (ws ? "gtrap_geebls_simple_ws" : "gtrap_geebls_simple")
+                          : (ws ? "gtrap_geebls_ws" : "gtrap_geebls");
   GTRAP_REQUIRE(blsp && lc && tu, "%s: null pointer", fn);
   GTRAP_REQUIRE(simple || (offset && ntrue), "%s: offset/ntrue required", fn);
   GTRAP_REQUIRE(nthread >= 1 && nthread <= 1024, "%s: nthread (block[0]) must be 1..1024", fn);
@@ -205,6 +271,14 @@ static int bls_launch(bool simple, float *blsp, uint32_t *phase1, uint32_t *phas
                 "%s: need 1 <= kma <= nthread (wrap, include/geebls.h:84-85)", fn);
   GTRAP_REQUIRE(kma <= nb, "%s: kma must not exceed nb", fn);
   GTRAP_REQUIRE(n >= 1 && nq >= 1 && nf >= 1 && nq <= 65535, "%s: bad sizes", fn);
+  if (ws) {
+    if (ws_bytes < 0 || (size_t)ws_bytes < bls_ws_need(n, nq, simple)) {
+      set_error("%s: workspace of %lld bytes, %zu needed", fn, (long long)ws_bytes,
+                bls_ws_need(n, nq, simple));
+      return GTRAP_ENOMEM;
+    }
+    GTRAP_REQUIRE(((uintptr_t)ws & 255) == 0, "%s: workspace must be 256-byte aligned", fn);
+  }
   const uint32_t nbk = nb + kma;
   const size_t per_f = (size_t)nbk * 12;
   int fpb = (int)((64 * 1024 - 4 * (size_t)nbk) / per_f);
@@ -217,14 +291,36 @@ static int bls_launch(bool simple, float *blsp, uint32_t *phase1, uint32_t *phas
   if (nused > n) nused = n;
   dim3 grid((nf + fpb - 1) / fpb, nq);
   const size_t shm = per_f * fpb + 4 * (size_t)nbk;
+  if (ws) {
+    unsigned char *w = static_cast<unsigned char *>(ws);
+    uint32_t *maxbits = reinterpret_cast<uint32_t *>(w);
+    float *lct = reinterpret_cast<float *>(w + bls_ws_part((size_t)nq * sizeof(uint32_t)));
+    float *tut = simple ? nullptr
+                        : reinterpret_cast<float *>(reinterpret_cast<unsigned char *>(lct) +
+                                                    bls_ws_part((size_t)n * nq * sizeof(float)));
+    hipLaunchKernelGGL(bls_zero_kernel, dim3((nq + 255) / 256), dim3(256), 0, st, maxbits, nq);
+    hipLaunchKernelGGL(bls_transpose_kernel,
+                       dim3((nq + kBtTile - 1) / kBtTile, (n + kBtTile - 1) / kBtTile),
+                       dim3(256), 0, st, lc, simple ? nullptr : tu, simple ? nullptr : offset,
+                       lct, tut, maxbits, n, nq);
+    if (simple)
+      hipLaunchKernelGGL((bls_kernel<true, true>), grid, dim3(kBlsThreads), shm, st, blsp,
+                         nullptr, nullptr, lct, tu, nullptr, nullptr, maxbits, n, nq, nf, nb,
+                         kma, kmi, kkmi, fmin, df, nused, fpb);
+    else
+      hipLaunchKernelGGL((bls_kernel<false, true>), grid, dim3(kBlsThreads), shm, st, blsp,
+                         phase1, phase2, lct, tut, offset, ntrue, maxbits, n, nq, nf, nb, kma,
+                         kmi, kkmi, fmin, df, nused, fpb);
+    return check_launch("bls_kernel");
+  }
   if (simple)
-    hipLaunchKernelGGL(bls_kernel<true>, grid, dim3(kBlsThreads), shm, st, blsp, nullptr,
-                       nullptr, lc, tu, nullptr, nullptr, n, nq, nf, nb, kma, kmi, kkmi, fmin,
-                       df, nused, fpb);
+    hipLaunchKernelGGL((bls_kernel<true, false>), grid, dim3(kBlsThreads), shm, st, blsp,
+                       nullptr, nullptr, lc, tu, nullptr, nullptr, nullptr, n, nq, nf, nb, kma,
+                       kmi, kkmi, fmin, df, nused, fpb);
   else
-    hipLaunchKernelGGL(bls_kernel<false>, grid, dim3(kBlsThreads), shm, st, blsp, phase1,
-                       phase2, lc, tu, offset, ntrue, n, nq, nf, nb, kma, kmi, kkmi, fmin, df,
-                       nused, fpb);
+    hipLaunchKernelGGL((bls_kernel<false, false>), grid, dim3(kBlsThreads), shm, st, blsp,
+                       phase1, phase2, lc, tu, offset, ntrue, nullptr, n, nq, nf, nb, kma, kmi,
+                       kkmi, fmin, df, nused, fpb);
   return check_launch("bls_kernel");
 }
 
@@ -236,7 +332,8 @@ extern "C" int gtrap_geebls(float *blsp, uint32_t *phase1, uint32_t *phase2, con
                             uint32_t kmi, uint32_t kkmi, float fmin, float df, uint32_t nthread,
                             void *stream) {
   return gtrap::bls_launch(false, blsp, phase1, phase2, lc, tu, offset, ntrue, n, nq, nf, nb,
-                           kma, kmi, kkmi, fmin, df, nthread, gtrap::as_stream(stream));
+                           kma, kmi, kkmi, fmin, df, nthread, nullptr, 0,
+                           gtrap::as_stream(stream));
 }
 
 extern "C" int gtrap_geebls_simple(float *blsp, const float *lc, const float *tu, uint32_t n,
@@ -244,5 +341,38 @@ extern "C" int gtrap_geebls_simple(float *blsp, const float *lc, const float *tu
                                    uint32_t kmi, uint32_t kkmi, float fmin, float df,
                                    uint32_t nthread, void *stream) {
   return gtrap::bls_launch(true, blsp, nullptr, nullptr, lc, tu, nullptr, nullptr, n, nq, nf,
-                           nb, kma, kmi, kkmi, fmin, df, nthread, gtrap::as_stream(stream));
+                           nb, kma, kmi, kkmi, fmin, df, nthread, nullptr, 0,
+                           gtrap::as_stream(stream));
+}
+
+extern "C" int64_t gtrap_geebls_ws_bytes(uint32_t n, uint32_t nq, int32_t simple) {
+  return (int64_t)gtrap::bls_ws_need(n, nq, simple != 0);
+}
+
+extern "C" int gtrap_geebls_ws(float *blsp, uint32_t *phase1, uint32_t *phase2, const float *lc,
+                               const float *tu, const float *offset, const uint32_t *ntrue,
+                               uint32_t n, uint32_t nq, uint32_t nf, uint32_t nb, uint32_t kma,
+                               uint32_t kmi, uint32_t kkmi, float fmin, float df,
+                               uint32_t nthread, void *ws, int64_t ws_bytes, void *stream) {
+  if (!ws) {
+    gtrap::set_error("gtrap_geebls_ws: null workspace");
+    return GTRAP_ENOMEM;
+  }
+  return gtrap::bls_launch(false, blsp, phase1, phase2, lc, tu, offset, ntrue, n, nq, nf, nb,
+                           kma, kmi, kkmi, fmin, df, nthread, ws, ws_bytes,
+                           gtrap::as_stream(stream));
+}
+
+extern "C" int gtrap_geebls_simple_ws(float *blsp, const float *lc, const float *tu, uint32_t n,
+                                      uint32_t nq, uint32_t nf, uint32_t nb, uint32_t kma,
+                                      uint32_t kmi, uint32_t kkmi, float fmin, float df,
+                                      uint32_t nthread, void *ws, int64_t ws_bytes,
+                                      void *stream) {
+  if (!ws) {
+    gtrap::set_error("gtrap_geebls_simple_ws: null workspace");
+    return GTRAP_ENOMEM;
+  }
+  return gtrap::bls_launch(true, blsp, nullptr, nullptr, lc, tu, nullptr, nullptr, n, nq, nf,
+                           nb, kma, kmi, kkmi, fmin, df, nthread, ws, ws_bytes,
+                           gtrap::as_stream(stream));
 }

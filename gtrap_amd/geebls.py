@@ -19,14 +19,53 @@ from . import _native
 from ._module import Module
 
 
+class Workspace:
+    """Grow-only device workspace of the curve-major BLS path
+    (``gtrap_geebls_ws_bytes``: the image copied curve-major plus one scale
+    per curve)."""
+
+    def __init__(self):
+        self.buf = None
+
+    def get(self, n: int, nq: int, simple: bool):
+        import torch
+        need = int(_native.load().gtrap_geebls_ws_bytes(int(n), int(nq), int(bool(simple))))
+        if self.buf is None or self.buf.numel() < need:
+            self.buf = torch.empty(need, dtype=torch.uint8, device="cuda")
+        return self.buf, need
+
+
+_WS: dict = {}
+
+
+def workspace_for(stream=None) -> Workspace:
+    """This module's default workspace for a stream (calls on one stream are
+    ordered, so they can share it)."""
+    import torch
+    key = (torch.cuda.current_device(), _native.stream_handle(stream))
+    if key not in _WS:
+        _WS[key] = Workspace()
+    return _WS[key]
+
+
 def geebls_device(blsp, phase1, phase2, lc, tu, offset, ntrue, n, nq, nf, nb, kma, kmi, kkmi,
-                  fmin, df, nthread=512, stream=None) -> None:
-    """Launch BLS on device buffers; interleaved (n, nq) lc/tu, outputs at jf + nf*q."""
+                  fmin, df, nthread=512, stream=None, workspace=None) -> None:
+    """Launch BLS on device buffers; interleaved (n, nq) lc/tu, outputs at jf + nf*q.
+
+    ``workspace`` (default: this module's per-stream one) selects the
+    curve-major path (``gtrap_geebls_ws``); ``workspace=False`` reads the
+    interleaved image in place (``gtrap_geebls``, no device memory beyond the
+    arguments).  Both give the same bits."""
     _native.require_gpu()
     p = _native.ptr
-    _native.call("gtrap_geebls", p(blsp), p(phase1), p(phase2), p(lc), p(tu), p(offset),
-                 p(ntrue), int(n), int(nq), int(nf), int(nb), int(kma), int(kmi), int(kkmi),
-                 float(fmin), float(df), int(nthread), _native.stream_handle(stream))
+    args = (p(blsp), p(phase1), p(phase2), p(lc), p(tu), p(offset), p(ntrue), int(n), int(nq),
+            int(nf), int(nb), int(kma), int(kmi), int(kkmi), float(fmin), float(df),
+            int(nthread))
+    if workspace is False:
+        _native.call("gtrap_geebls", *args, _native.stream_handle(stream))
+        return
+    buf, need = (workspace or workspace_for(stream)).get(n, nq, False)
+    _native.call("gtrap_geebls_ws", *args, p(buf), int(need), _native.stream_handle(stream))
 
 
 def geebls_f64_device(sr, phase1, phase2, lc, tu, t64, offset, ntrue, n, nq, nf, nb, kma, kmi,

@@ -12,11 +12,19 @@ from ._module import Module
 
 
 def geebls_simple_device(blsp, lc, tu, n, nq, nf, nb, kma, kmi, kkmi, fmin, df, nthread=512,
-                         stream=None) -> None:
+                         stream=None, workspace=None) -> None:
+    """``workspace`` as in :func:`gtrap_amd.geebls.geebls_device` (default: the
+    per-stream one; ``False``: the interleaved image in place)."""
+    from .geebls import workspace_for
     _native.require_gpu()
     p = _native.ptr
-    _native.call("gtrap_geebls_simple", p(blsp), p(lc), p(tu), int(n), int(nq), int(nf),
-                 int(nb), int(kma), int(kmi), int(kkmi), float(fmin), float(df), int(nthread),
+    args = (p(blsp), p(lc), p(tu), int(n), int(nq), int(nf), int(nb), int(kma), int(kmi),
+            int(kkmi), float(fmin), float(df), int(nthread))
+    if workspace is False:
+        _native.call("gtrap_geebls_simple", *args, _native.stream_handle(stream))
+        return
+    buf, need = (workspace or workspace_for(stream)).get(n, nq, True)
+    _native.call("gtrap_geebls_simple_ws", *args, p(buf), int(need),
                  _native.stream_handle(stream))
 
 

@@ -105,6 +105,29 @@ int gtrap_geebls_simple(float *blsp, const float *lc, const float *tu,
                         float fmin, float df, uint32_t nthread, void *stream);
 
 /*
+ * gtrap_geebls and gtrap_geebls_simple with a caller workspace: no
+ * counterpart in the reference (whose kernel reads the interleaved image in
+ * place, include/geebls.h:60-77); for callers that can lend device memory.
+ * One pass copies the interleaved (n, nq) image to curve-major order in the
+ * workspace and takes each curve's fixed-point scale once, so the binning
+ * loads are coalesced.  Results are bit-identical to the entry points above.
+ * ws: at least gtrap_geebls_ws_bytes(n, nq, simple) bytes, 256-byte aligned;
+ * GTRAP_ENOMEM when smaller.  Calls that share a workspace must not overlap.
+ */
+int64_t gtrap_geebls_ws_bytes(uint32_t n, uint32_t nq, int32_t simple);
+int gtrap_geebls_ws(float *blsp, uint32_t *phase1, uint32_t *phase2,
+                    const float *lc, const float *tu, const float *offset,
+                    const uint32_t *ntrue, uint32_t n, uint32_t nq, uint32_t nf,
+                    uint32_t nb, uint32_t kma, uint32_t kmi, uint32_t kkmi,
+                    float fmin, float df, uint32_t nthread, void *ws, int64_t ws_bytes,
+                    void *stream);
+int gtrap_geebls_simple_ws(float *blsp, const float *lc, const float *tu,
+                           uint32_t n, uint32_t nq, uint32_t nf, uint32_t nb,
+                           uint32_t kma, uint32_t kmi, uint32_t kkmi,
+                           float fmin, float df, uint32_t nthread, void *ws,
+                           int64_t ws_bytes, void *stream);
+
+/*
  * Arithmetic variants of the geebls kernel (include/geebls.h:1-167) for
  * precision studies; the search itself is unchanged.  variant bits:
  *   bit 0      flux read as binary16 lch[(n, nq)] = (lc - offset) * 2^k per

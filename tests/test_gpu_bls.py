@@ -182,3 +182,64 @@ def test_invalid_arguments_raise(gpu):
         _run_geebls(lc, tu, off, ntrue, 4, 1000, 8, 1, 5, 0.1, 0.01, 500)  # nb not pow2
     with pytest.raises(ValueError):
         _run_geebls(lc, tu, off, ntrue, 4, 1024, 600, 1, 5, 0.1, 0.01, 512)  # kma > nthread
+
+
+def test_workspace_path_matches_in_place(gpu):
+    """The curve-major path (gtrap_geebls_ws / _simple_ws: image copied through
+    the workspace, one scale per curve) returns the bits of the in-place entry
+    points, on ragged sizes (two 64-curve transpose tiles, a partial sample
+    tile), with NaN fluxes and masked times in the batch."""
+    import torch
+    from gtrap_amd import geebls, geebls_simple
+    lc, tu, off, ntrue = _small(n=3001, nq=67, seed=11)
+    lc = lc.copy()
+    lc[17, 3] = np.nan
+    lc[:, 5] = np.nan  # a curve of NaN: scale from 0, as in place
+    nf, nb, kma, kmi, nthread = 41, 512, 20, 4, 256
+    kkmi = int(3001 * kmi / nb)
+    outs = []
+    for ws in (False, None, geebls.Workspace()):
+        blsp = torch.full((67 * nf,), -1.0, dtype=torch.float32, device="cuda")
+        p1 = torch.zeros(67 * nf, dtype=torch.int32, device="cuda")
+        p2 = torch.zeros(67 * nf, dtype=torch.int32, device="cuda")
+        geebls.geebls_device(blsp, p1, p2, _dev(lc), _dev(tu), _dev(off), _dev(ntrue), 3001, 67,
+                             nf, nb, kma, kmi, kkmi, 1.0 / 30, 0.011, nthread=nthread,
+                             workspace=ws)
+        torch.cuda.synchronize()
+        outs.append([x.cpu().numpy() for x in (blsp, p1, p2)])
+    for o in outs[1:]:
+        for a, b, name in zip(outs[0], o, ("blsp", "phase1", "phase2")):
+            np.testing.assert_array_equal(a, b, err_msg=name)
+    # against the oracle on the curves without NaN flux (NaN sums have no
+    # defined value in the reference's float atomics)
+    want = ob.geebls(lc, tu, off, ntrue, nf, nb, kma, kmi, kkmi, 1.0 / 30, 0.011, nthread=nthread)
+    fin = np.isfinite(lc).all(0)
+    np.testing.assert_array_equal(outs[1][0].reshape(67, nf)[fin], want[0][fin])
+    # simple: common 1-D time, pre-centred flux
+    t = (np.arange(3001) * 0.0204).astype(np.float32)
+    lcs = (lc - off[None, :]).astype(np.float32)
+    res = []
+    for ws in (False, None):
+        blsp = torch.zeros(67 * nf, dtype=torch.float32, device="cuda")
+        geebls_simple.geebls_simple_device(blsp, _dev(lcs), _dev(t), 3001, 67, nf, nb, kma, kmi,
+                                           kkmi, 0.05, 0.013, nthread=nthread, workspace=ws)
+        torch.cuda.synchronize()
+        res.append(blsp.cpu().numpy())
+    np.testing.assert_array_equal(res[0], res[1])
+
+
+def test_workspace_too_small_is_an_error(gpu):
+    import torch
+    from gtrap_amd import _native
+    lc, tu, off, ntrue = _small(n=512, nq=2)
+    need = int(_native.load().gtrap_geebls_ws_bytes(512, 2, 0))
+    ws = torch.empty(need, dtype=torch.uint8, device="cuda")
+    blsp = torch.zeros(2 * 4, dtype=torch.float32, device="cuda")
+    p = _native.ptr
+    dl, dt, do, dn = _dev(lc), _dev(tu), _dev(off), _dev(ntrue)  # held across the launches
+    args = (p(blsp), None, None, p(dl), p(dt), p(do), p(dn), 512, 2, 4, 256, 8, 1, 5, 0.1, 0.01,
+            256)
+    with pytest.raises(_native.GtrapError):
+        _native.call("gtrap_geebls_ws", *args, p(ws), need - 1, None)
+    _native.call("gtrap_geebls_ws", *args, p(ws), need, None)
+    torch.cuda.synchronize()
