@@ -19,6 +19,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+LDS_PEAK_TBS = 256 * 128 * 2.4e9 / 1e12  # 128 B/clk per CU, 256 CUs, 2.4 GHz
 FP64_PEAK = 78.6  # TFLOP/s, MI355X dense FP64
 # FP64 operations of one (epoch, W, L) trial from the tile prefix sums: the
 # region sums from prefix differences, shifted to epoch coordinates for the
@@ -169,7 +170,7 @@ def bench_geebls(steps: int, warmup: int):
         nfs += 16
     cpu_s_per_curve = (time.perf_counter() - t1) / nfs * nf
     ach = lds_bytes / (kms * 1e-3) / 1e12
-    lds_peak = 256 * 128 * 2.4e9 / 1e12
+    lds_peak = LDS_PEAK_TBS
     return dict(
         metric="light curves/s and trial evals/s (geebls + getstat)",
         value=nq * steps / wall, unit="light curves/s", ms_per_step=wall / steps * 1e3,
@@ -318,7 +319,7 @@ def bench_precision(steps: int, warmup: int):
     kms = own["ms"]
     lds_bytes = nf * nq * (12.0 * n + 8.0 * (g["nb"] + g["kma"]))
     ach = lds_bytes / (kms * 1e-3) / 1e12
-    lds_peak = 256 * 128 * 2.4e9 / 1e12
+    lds_peak = LDS_PEAK_TBS
     return dict(
         metric="precision study: SR / SDE error vs float64 (geebls arithmetic variants)",
         value=own["sde_rel_max"], unit="max relative SDE error (this build's arithmetic)",
@@ -369,8 +370,92 @@ def bench_copy(steps: int, warmup: int):
                 cpu_baseline=None)
 
 
+# ----------------------------------------------------------------------------
+# gtls workload: the single-transit-epoch search at the reference driver's
+# Kepler setting (kepler/gtls_mockkepler.py:240-260): 72000 samples at
+# 0.0204 d, nw = 1024 widths over 1-6 d, nl = 20, nsc = int(wmax/deltat + 3)
+# ----------------------------------------------------------------------------
+GTLS_NQ = 4
+
+
+def gtls_lds_bytes_per_trial(widths, nsc: int, nl: int, deltat: float) -> float:
+    """LDS bytes the sequential scan of one (epoch, W, L) trial reads (4 B
+    floats): the skip scan up to -W/2 reads times (nsc/2 - W/2/deltat), the
+    region scans read time and flux over the window (2 W/deltat), the residual
+    pass reads them again (2 W/deltat); the per-W mean pass (2 nsc) is shared
+    by the nl trials.  Mean over the width grid."""
+    w = np.asarray(widths, np.float64) / deltat
+    skip = np.maximum(0.0, nsc / 2.0 - w / 2.0)
+    return float(np.mean(4.0 * (skip + 4.0 * w + 2.0 * nsc / nl)))
+
+
+def bench_gtls(steps: int, warmup: int):
+    import torch
+    from gtrap_amd import gtls
+    from oracle import gtls as og
+    n, nq, nw, nl = 72000, GTLS_NQ, 1024, 20
+    wmin, wmax = 1.0, 6.0
+    rng = np.random.default_rng(1006)
+    t = np.arange(n) * 0.0204
+    tu = np.repeat(t[:, None], nq, 1)
+    tu[rng.random((n, nq)) < 0.02] = -1000.0
+    x = 1.0 + 1e-3 * rng.standard_normal((n, nq))
+    from gtrap_amd import synth
+    for q in range(nq):  # one transit-like dip per curve
+        x[:, q] -= synth.trapezoid_dip(t, rng.uniform(100, 1300), None, 3e-3, 0.4, 0.04)
+    lc = (2.0 - x).astype(np.float32)  # transit flip, kepler/gtls_mockkepler.py:213-215
+    tu = tu.astype(np.float32)
+    gp = gtls.gtls_params(tu[:, 0], wmin, wmax, nw, nl)
+    deltat, dw, nsc = float(gp["deltat"]), float(gp["dw"]), int(gp["nsc"])
+    dl, dt = (torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (lc, tu))
+    outs = [torch.empty(n * nq, dtype=torch.float32, device="cuda") for _ in range(5)]
+    log(f"gtls: {nq} curves x {n} samples, nw {nw}, nl {nl}, nsc {nsc}")
+
+    def run():
+        gtls.gtls_device(*outs, dl, dt, n, nq, nw, nl, nsc, wmax, dw, deltat)
+
+    for _ in range(warmup):
+        run()
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    a.record()
+    for _ in range(steps):
+        run()
+    b.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kms = a.elapsed_time(b) / steps
+    trials = float(n) * nq * nw * nl
+    widths = wmax - dw * np.arange(nw)
+    bpt = gtls_lds_bytes_per_trial(widths, nsc, nl, deltat)
+    ach = trials * bpt / (kms * 1e-3) / 1e12
+    log(f"gtls kernel {kms:.2f} ms; trials/s {trials / kms * 1e3:.3e}; {bpt:.0f} LDS B/trial")
+    # CPU: the C restatement over a bounded slice of epochs of one curve
+    cores, model = cpu_info()
+    k0, nk = n // 2, 600  # about 13 s of one core
+    c0 = time.perf_counter()
+    og.gtls(lc[:, :1], tu[:, :1], nw, nl, nsc, wmax, dw, deltat, ks=(k0, k0 + nk))
+    csec = time.perf_counter() - c0
+    cpu_lcs = 1.0 / (csec * n / nk)
+    return dict(metric="light curves/s and (epoch, W, L) trials/s (gtls single-transit search)",
+                value=nq / (kms * 1e-3), unit="light curves/s", ms_per_step=wall / steps * 1e3,
+                dtype="f32", trials_per_s=trials / (kms * 1e-3),
+                config=dict(workload=f"gtls normal mode, {nq} curves x {n} samples at 0.0204 d "
+                                     f"(2% masked), nw {nw} widths 1-6 d, nl {nl}, nsc {nsc}",
+                            n=n, nq=nq, nw=nw, nl=nl, nsc=nsc),
+                roofline=dict(bound="lds", achieved=ach, peak=LDS_PEAK_TBS, unit="TB/s",
+                              frac=ach / LDS_PEAK_TBS, traffic=None, kernel="gtls_kernel",
+                              kernel_ms=kms, lds_bytes_per_trial=bpt),
+                cpu_baseline=dict(value=cpu_lcs, unit="light curves/s", cores=1, kind="port",
+                                  trials_per_s=cpu_lcs * n * nw * nl,
+                                  sample=f"{nk} epochs of one curve with the C oracle, "
+                                         f"scaled to {n} epochs ({model})"))
+
+
 WORKLOADS = {"gfilter": bench_gfilter, "geebls": bench_geebls, "serve": bench_serve,
-             "copy": bench_copy, "precision": bench_precision}
+             "copy": bench_copy, "precision": bench_precision, "gtls": bench_gtls}
 
 
 def main(argv=None):
