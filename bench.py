@@ -380,13 +380,13 @@ GTLS_NQ = 4
 
 def gtls_lds_bytes_per_trial(widths, nsc: int, nl: int, deltat: float) -> float:
     """LDS bytes the sequential scan of one (epoch, W, L) trial reads (4 B
-    floats): the skip scan up to -W/2 reads times (nsc/2 - W/2/deltat), the
-    region scans read time and flux over the window (2 W/deltat), the residual
-    pass reads them again (2 W/deltat); the per-W mean pass (2 nsc) is shared
-    by the nl trials.  Mean over the width grid."""
+    floats): the region scans read time and flux over the window
+    (2 W/deltat), the residual pass reads them again (2 W/deltat); the per-W
+    mean pass (2 nsc) and skip scan up to -W/2 (nsc/2 - W/2/deltat times) are
+    shared by the nl trials.  Mean over the width grid."""
     w = np.asarray(widths, np.float64) / deltat
     skip = np.maximum(0.0, nsc / 2.0 - w / 2.0)
-    return float(np.mean(4.0 * (skip + 4.0 * w + 2.0 * nsc / nl)))
+    return float(np.mean(4.0 * (4.0 * w + (2.0 * nsc + skip) / nl)))
 
 
 def bench_gtls(steps: int, warmup: int):

@@ -42,6 +42,10 @@ __device__ void gtls_thread(const float *cache, int nsc, int nl, float W, float 
     }
   }
   if (nsum > 0) mean = mean / nsum;
+  // samples before -W/2 (the reference recounts them for every L; the count
+  // does not depend on L)
+  int nSx = 0;
+  for (int i = 0; i < nsc; ++i) { if (ct[i] > tStart) break; nSx++; }
   res_max = 0.0f;
   Lmax = 0.0f;
   Hmaxmax = 0.0f;
@@ -49,11 +53,10 @@ __device__ void gtls_thread(const float *cache, int nsc, int nl, float W, float 
     const float L = (mode ? (float)(mm + 1) : (float)mm) * dl;
     float xAEC = 0, xBD = 0, xtB = 0, xtD = 0, t2BD = 0, tB = 0, tD = 0;
     int nA = 0, nB = 0, nC = 0, nD = 0, nE = 0;
-    int nSx = 0, nAx = 0, nBx = 0, nCx = 0, nDx = 0, nEx = 0;
+    int nAx = 0, nBx = 0, nCx = 0, nDx = 0, nEx = 0;
     const float tAB = (float)((double)(-W) * 0.25 - (double)L * 0.5);
     const float tBC = (float)((double)(-W) * 0.25 + (double)L * 0.5);
     const float tCD = -tBC, tDE = -tAB;
-    for (int i = 0; i < nsc; ++i) { if (ct[i] > tStart) break; nSx++; }
     for (int i = 0; i < nsc - nSx; ++i) {
       const float tnow = ct[i + nSx];
       if (tnow > tAB) break;
