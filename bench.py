@@ -454,8 +454,93 @@ def bench_gtls(steps: int, warmup: int):
                                          f"scaled to {n} epochs ({model})"))
 
 
+# ----------------------------------------------------------------------------
+# tlsfit workload: the reference's precise refit (works/sp/gtls_tesslc.py:
+# 299-314: 10 t0 x 10 W x 20 L around a candidate) on a 27.4 d TESS sector at
+# 2 min cadence, through gtrap_amd.tls.tlsfit; and a 1e6-trial scan of the
+# same curve for the kernel's rate
+# ----------------------------------------------------------------------------
+# float64 operations per visited window sample of one trial in tlsgrid_kernel:
+# mean pass 3 (offset, sum, count), region pass 5 (offset, centring and the
+# region sums, B/D's time moments averaged in), residual pass 6
+TLSGRID_FLOPS_PER_SAMPLE = 14
+TLSFIT_REFITS = 100
+
+
+def bench_tlsfit(steps: int, warmup: int):
+    import torch
+    from gtrap_amd import tls
+    from oracle import tlsfold as ot
+    rng = np.random.default_rng(1009)
+    n = int(27.4 * 720)
+    t = np.arange(n) * (2.0 / 1440.0)
+    x = 1.0 + 5e-4 * rng.standard_normal(n) + tls.gen_trapzoid(t, -2e-3, 0.3, 0.05, 13.7)
+    x[rng.random(n) < 0.01] = np.nan
+    cands = rng.uniform(1.0, 26.4, TLSFIT_REFITS)
+
+    def grids(T0p, Wp=0.3):
+        tl = np.linspace(T0p - 0.1, T0p + 0.1, 10)
+        Wl = np.linspace(Wp - Wp / 100, Wp + Wp / 100, 10)
+        Ll = np.linspace(0.0, Wp / 2.0, 20)
+        return Wl, Ll, tl
+
+    for _ in range(max(1, warmup)):
+        tls.tlsfit(t, x, *grids(cands[0]))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for c in cands:
+            tls.tlsfit(t, x, *grids(c))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    refit_ms = wall / (steps * TLSFIT_REFITS) * 1e3
+    # kernel rate: 10 W x 20 L x 5000 t0 over the curve
+    Wl, Ll = np.linspace(0.1, 0.5, 10), np.linspace(0.0, 0.2, 20)
+    tl = np.linspace(0.2, 27.2, 5000)
+    tls.grid_device(t, x, Wl, Ll, tl)
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    tls.grid_device(t, x, Wl, Ll, tl)
+    b.record()
+    torch.cuda.synchronize()
+    kms = a.elapsed_time(b)
+    ntr = len(Wl) * len(Ll) * len(tl)
+    win = sum(len(Ll) * float(np.sum(np.searchsorted(t, tl + w / 2, "right")
+                                     - np.searchsorted(t, tl - w / 2, "left"))) for w in Wl)
+    ach = TLSGRID_FLOPS_PER_SAMPLE * win / (kms * 1e-3) / 1e12
+    log(f"tlsfit refit {refit_ms:.3f} ms; scan kernel {kms:.3f} ms ({ntr} trials)")
+    # CPU: the numpy restatement of get_chi2Hmax over 15 refits (about 10 s)
+    cores, model = cpu_info()
+    ncpu = 15
+    c0 = time.perf_counter()
+    for c in cands[:ncpu]:
+        Wc, Lc, tc = grids(c)
+        for w in Wc:
+            for l in Lc:
+                for k in tc:
+                    ot.chi2hmax_numpy(t - k, x, w, l)
+    csec = (time.perf_counter() - c0) / ncpu
+    return dict(metric="candidate refits/s (tls.py tlsfit, 2000 trials each) and trials/s",
+                value=1e3 / refit_ms, unit="refits/s", ms_per_step=wall / steps * 1e3,
+                dtype="f64", refit_ms=refit_ms, scan_trials_per_s=ntr / (kms * 1e-3),
+                config=dict(workload=f"tlsfit refits of 10 t0 x 10 W x 20 L on a 27.4 d sector "
+                                     f"at 2 min ({n} samples, 1% NaN), {TLSFIT_REFITS} "
+                                     f"candidates per step; scan 10 W x 20 L x 5000 t0",
+                            n=n, refits_per_step=TLSFIT_REFITS),
+                roofline=dict(bound="fp64", achieved=ach, peak=FP64_PEAK, unit="TFLOP/s",
+                              frac=ach / FP64_PEAK, traffic=None, kernel="tlsgrid_kernel (scan)",
+                              kernel_ms=kms, window_samples_per_trial=win / ntr,
+                              flops_per_window_sample=TLSGRID_FLOPS_PER_SAMPLE),
+                cpu_baseline=dict(value=1.0 / csec, unit="refits/s", cores=1, kind="port",
+                                  sample=f"{ncpu} refits of 2000 trials with the numpy "
+                                         f"restatement of get_chi2Hmax ({model})"))
+
+
 WORKLOADS = {"gfilter": bench_gfilter, "geebls": bench_geebls, "serve": bench_serve,
-             "copy": bench_copy, "precision": bench_precision, "gtls": bench_gtls}
+             "copy": bench_copy, "precision": bench_precision, "gtls": bench_gtls,
+             "tlsfit": bench_tlsfit}
 
 
 def main(argv=None):

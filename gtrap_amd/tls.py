@@ -44,9 +44,11 @@ def grid_device(t, x, Wl, Ll, tl, sigma=1.0, divx=True, stream=None):
     nn = torch.empty(ntr, dtype=torch.int32, device="cuda")
     xs = torch.empty(ntr, dtype=torch.float64, device="cuda")
     p = _native.ptr
+    tt = np.asarray(t, np.float64).reshape(-1)
+    srt = bool(tt.size < 2 or np.all(tt[1:] >= tt[:-1]))  # NaN compares false: unsorted
     _native.call("gtrap_tlsgrid", p(t_), p(x_), int(t_.numel()), p(W_), shape[0], p(L_),
-                 shape[1], p(T_), shape[2], 1 if divx else 0, float(sigma), p(H), p(res), p(nn),
-                 p(xs), _native.stream_handle(stream))
+                 shape[1], p(T_), shape[2], 1 if divx else 0, float(sigma), int(srt), p(H),
+                 p(res), p(nn), p(xs), _native.stream_handle(stream))
     return (H.view(shape), res.view(shape), nn.view(shape), xs.view(shape))
 
 
