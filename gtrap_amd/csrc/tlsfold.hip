@@ -7,23 +7,27 @@
 //   d = u - t0, or (u - P) - t0 if d >= P/2, or (u + P) - t0 if d < -P/2;
 //   L = m * (0.45*W/nl); regions A..E of tls.py:175-179 on d; mean over
 //   [-W/2, W/2]; a trial needs n > 3 samples in [-W/2, W/2) (n - 3 degrees
-//   of freedom), else it is NaN; best trial per period = first maximum in
-//   (W, L, epoch) order with NaN skipped (tlsfit, tls.py:217-231).
+//   of freedom) and W < P (a window as long as the period would hold two
+//   images of a sample), else it is NaN; best trial per period = first
+//   maximum in (W, L, epoch) order with NaN skipped (tlsfit, tls.py:217-231).
 //
-// MI355X design.  One workgroup owns one (curve, period) work item at a time
-// (persistent grid, per-block global workspace):
-//  1. fold + bucket sort: u of every valid sample is bucketed into ~n/2
-//     uniform phase buckets (global atomics), scanned, scattered, and each
-//     bucket insertion-sorted, giving the folded curve sorted by phase;
-//  2. epoch tiles: the union of the windows [t0 - Wmax/2, t0 + Wmax/2] of a
-//     tile of epochs is a contiguous range of the (cyclically extended)
-//     sorted curve; it is staged in LDS with float64 prefix sums of
-//     (x', x'^2, q, q^2, x'q), q = phase relative to the tile, x' = x - xref;
-//  3. one thread per (epoch, W): region boundaries are found by binary search
-//     with the exact float64 offset test, region sums are prefix differences
-//     shifted to the epoch's coordinates, and each L gives H, res and S/N in
-//     O(1).  Windows larger than the LDS tile are swept in chunks with the
-//     boundary sums accumulated per (W, boundary).
+// MI355X design.  One 512-thread workgroup owns one (curve, period) work item
+// at a time (persistent grid, atomic ticket, per-block global workspace):
+//  1. fold + bucket sort: u of every valid sample is counted into ~n/2
+//     phase buckets (LDS atomics), scanned, scattered to the workspace as
+//     16-byte records, and each bucket is ordered by (u, flux bits) in LDS
+//     chunks, giving the folded curve sorted by phase;
+//  2. epoch tiles: the union of the windows [t0 - Wmax/2, t0 + Wmax/2] of up
+//     to 96 epochs is a contiguous range of the (cyclically extended) sorted
+//     curve; it is staged in LDS with float64 prefix sums of
+//     (x', x'^2, q, q^2, x'q), q = phase relative to the tile's central
+//     epoch, x' = x - xref, and a two-cells-per-sample phase index;
+//  3. one thread per (epoch, W, L) trial: the four region boundaries come from
+//     the phase index and a fixed-depth branch-free bisection with the exact
+//     float64 offset test; region sums are prefix differences shifted to the
+//     epoch's coordinates; trials are ranked by sign(S/N)*(S/N)^2 (one
+//     division).  Windows larger than the LDS tile are swept in chunks with
+//     the boundary sums accumulated per (epoch, W, boundary).
 #include "common.h"
 #include <math.h>
 
@@ -841,6 +845,9 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
             p0 = tf_clamp(p0, 0, cnt);
             pE = tf_clamp(pE, p0, cnt);
             pM = tf_clamp(pM, pE, cnt);
+            // W >= P is no trial (the window would hold two images of a
+            // sample): an empty window makes every trial of it NaN
+            if (!(W < P)) p0 = pE = pM = 0;
             ewp[3 * k] = p0; ewp[3 * k + 1] = pE; ewp[3 * k + 2] = pM;
             tf_window((double)(pM - p0), L.px[pM][0] - L.px[p0][0], (double)(pE - p0),
                       L.px[pE][0] - L.px[p0][0], L.px[pE][1] - L.px[p0][1], ewd[2 * k],
@@ -960,10 +967,12 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
             const double *v = base + 6 * b;
             return TfSums{v[0], v[1], v[2], v[3], v[4], v[5]};
           };
-          double nB, dA;
-          const double key = tf_stat(get(0), get(3 + 4 * mm), get(4 + 4 * mm), get(5 + 4 * mm),
-                                     get(6 + 4 * mm), get(1), get(2), W,
-                                     wlt + kTfWl * (w * nl + m), &nB, &dA);
+          double nB = NAN, dA = NAN;
+          const double key =
+              W < P ? tf_stat(get(0), get(3 + 4 * mm), get(4 + 4 * mm), get(5 + 4 * mm),
+                              get(6 + 4 * mm), get(1), get(2), W, wlt + kTfWl * (w * nl + m),
+                              &nB, &dA)
+                    : __builtin_nan("");  // W >= P: no trial
           if (tf_better(key, w, m, e0 + el, best)) best = TfBest{key, nB, dA, w, m, e0 + el};
         }
         __syncthreads();

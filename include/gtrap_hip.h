@@ -287,7 +287,8 @@ int gtrap_tlsgrid(const double *t, const double *x, int32_t n, const double *Wl,
  * curve: u = fmod(t, P), epochs e < ceil(P/dtep) at t0 = e*dtep, offset
  * d = u - t0 wrapped by +-P to [-P/2, P/2); widths W = widths[w] and
  * L = m*(0.45*W/nl), m < nl (the L grid of include/gtls_simple.h:69,167).
- * Samples with t < 0 are masked.  Best trial per period = first maximum in
+ * Samples with t < 0 are masked.  A trial with n <= 3 samples in its window
+ * or with W >= P is NaN.  Best trial per period = first maximum in
  * (w, m, e) order, NaN skipped.
  *
  * Ragged batch: curve c has len[c] samples at offset off[c] of the flat t, x

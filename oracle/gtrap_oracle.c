@@ -170,7 +170,8 @@ void orc_geebls(float *blsp, uint32_t *phase1, uint32_t *phase2, const float *lc
 /* D [W/4-L/2, W/4+L/2), E [W/4+L/2, W/2); mean over [-W/2, W/2] (:187);     */
 /* denA, numB, H = -numB/denA (:200-202); res (:204-206);                   */
 /* S/N = -H / sqrt(res / (n - 3)) (tls.py:100); a trial with n <= 3 (no      */
-/* degrees of freedom) is NaN.                                              */
+/* degrees of freedom) is NaN, and so is one with W >= P (a window as long  */
+/* as the period would hold two images of a sample).                        */
 /* ------------------------------------------------------------------------ */
 static double fold_offset(double u, double t0, double P) {
   double d = u - t0;
@@ -229,6 +230,7 @@ double orc_tlsfold_trial(const float *t, const float *x, int n, double P, int e,
   if (Hout) *Hout = H;
   if (cnt) *cnt = nn;
   if (nn <= 3) return NAN; /* n - 3 degrees of freedom: no trial below 4 samples */
+  if (!(W < P)) return NAN; /* no trial: the window spans the period */
   return -H / sqrt(res / (double)(nn - 3));
 }
 
@@ -321,18 +323,27 @@ void orc_tlsfold(const float *t, const float *x, int n, const double *periods, i
        * the sorted order only narrows the scan (1e-9 d margin), the test
        * itself uses the exact offset */
       int kk = 0;
-      for (int img = -1; img <= 1; ++img) {
-        const double lo = t0 - hwm - 1e-9 - img * P, hi = t0 + hwm + 1e-9 - img * P;
-        for (int i = lower_u(s, k, lo); i < k && s[i].u <= hi; ++i) {
+      if (2.0 * (hwm + 1e-9) < P) {
+        /* three disjoint image ranges of the sorted phases */
+        for (int img = -1; img <= 1; ++img) {
+          const double lo = t0 - hwm - 1e-9 - img * P, hi = t0 + hwm + 1e-9 - img * P;
+          for (int i = lower_u(s, k, lo); i < k && s[i].u <= hi; ++i) {
+            const double dd = fold_offset(s[i].u, t0, P);
+            if (dd >= -hwm && dd <= hwm) { d[kk] = dd; xv[kk] = s[i].x; kk++; }
+          }
+        }
+      } else {
+        /* the ranges overlap (Wmax >= P): every sample once, at its offset */
+        for (int i = 0; i < k; ++i) {
           const double dd = fold_offset(s[i].u, t0, P);
-          /* the three image ranges are disjoint because Wmax < P */
           if (dd >= -hwm && dd <= hwm) { d[kk] = dd; xv[kk] = s[i].x; kk++; }
         }
       }
       for (int w = 0; w < nw; ++w) {
         for (int m = 0; m < nl; ++m) {
           double H;
-          const double sn = trial_window(d, xv, kk, widths[w], m, nl, &H);
+          const double sn = widths[w] < P ? trial_window(d, xv, kk, widths[w], m, nl, &H)
+                                          : (H = NAN, NAN);
           const int earlier = (w < bw) || (w == bw && (m < bl || (m == bl && e < be)));
           if (sn > bsn || (sn == bsn && earlier)) { bsn = sn; bw = w; bl = m; be = e; bh = H; }
         }

@@ -8,7 +8,8 @@ by (u + P) - t0 when d < -P/2.  For each width W and L = m*(0.45*W/nl)
 (m < nl; the reference's normal-mode grid, include/gtls_simple.h:69,167) the
 statistic of gtrap/tls.py:173-208 (get_chi2Hmax with divx=True, sigma=1) is
 evaluated on d, and S/N = -H/sqrt(res/(n-3)) (tls.py:229); a trial with
-n <= 3 samples in [-W/2, W/2) has no degrees of freedom and is NaN.  The best trial of
+n <= 3 samples in [-W/2, W/2) has no degrees of freedom and is NaN, and so is
+a trial with W >= P (its window would hold two images of a sample).  The best trial of
 a period is the first maximum in (W, L, epoch) order, NaN skipped
 (tls.py:217-231, nanargmax).
 

@@ -80,3 +80,35 @@ def test_single_curve_grid_matches_oracle(gpu):
     got = tlsfold.tlsfold([(d["t"], d["flux"])], [per], d["widths"], 10, 0.0204)[0]
     want = ot.tlsfold(d["t"], d["flux"], per, d["widths"], 10, 0.0204)
     assert int(np.nanargmax(got["sn"])) == int(np.nanargmax(want["sn"]))
+
+
+def test_degenerate_curves_and_periods(gpu):
+    """Curves with 3, 2, 1 and no valid samples (no trial: S/N -inf, indices
+    -1); periods below one epoch step and below the widest window (W >= P is
+    no trial); a period commensurate with the cadence (phases in a few tight
+    clusters, large brackets); exact duplicate phases with different fluxes."""
+    rng = np.random.default_rng(21)
+    widths = np.linspace(0.04, 0.3, 10)
+    tiny = []
+    for n, masked in ((3, 0), (10, 8), (1, 0), (6, 6)):
+        t = (np.arange(n) * 0.0204).astype(np.float32)
+        x = (1.0 + 1e-3 * rng.standard_normal(n)).astype(np.float32)
+        t[:masked] = -1000.0
+        tiny.append((t, x))
+    _check(tiny, [np.array([0.5, 0.03, 2.0])] * 4, widths, 10)
+    t, x = _curve(3000, 22)
+    _check([(t, x)], [np.array([0.015, 0.2, 0.25, 0.31, 0.7])], np.array([0.1, 0.3, 0.6]), 6)
+    n = 6000
+    tc = (np.arange(n) * 0.0204).astype(np.float32)
+    xc = (1.0 + 1e-3 * rng.standard_normal(n)).astype(np.float32)
+    _check([(tc, xc)], [np.array([0.0204 * 7, 0.0204 * 25, 0.0204 * 49])], widths, 10)
+    td = (np.arange(400) * 0.25).astype(np.float32)  # phases exactly 0, 0.25, ...
+    xd = (1.0 + 1e-3 * rng.standard_normal(400)).astype(np.float32)
+    _check([(td, xd)], [np.array([0.5, 1.0, 0.75])], np.array([0.1, 0.2, 0.4]), 5)
+
+
+def test_longest_serving_curve(gpu):
+    """A curve of the serving maximum (70000 samples, 2% masked) at a chunked
+    short period, a middle one and a long one (few cycles per window)."""
+    t, x = _curve(70000, 23, mask=0.02)
+    _check([(t, x)], [np.array([0.6, 23.3, 466.0])], np.linspace(0.04, 0.3, 10), 10)

@@ -103,3 +103,45 @@ def test_recovers_injected_periodic_trapezoid():
     assert int(np.argmax(r["sn"])) == 2
     assert r["w"][2] == 1 and abs(r["e"][2] * 0.0204 - t0) < 0.03
     assert r["h"][2] < 0
+
+
+def _brute(t, x, periods, widths, nl, dtep):
+    """Best trial per period by evaluating every trial with `trial` (each
+    sample folded once, no candidate gathering)."""
+    out = []
+    for P in periods:
+        best = (-np.inf, -1, -1, -1)
+        for w, W in enumerate(widths):
+            for m in range(nl):
+                for e in range(int(np.ceil(P / dtep))):
+                    sn = ot.trial(t, x, P, e, W, m, nl, dtep)[0]
+                    if sn > best[0]:
+                        best = (sn, w, m, e)
+        out.append(best)
+    return out
+
+
+def test_window_as_long_as_the_period_is_no_trial():
+    """W >= P is NaN in the direct trial; the candidate gathering, whose image
+    ranges overlap when the widest window reaches the period, still counts
+    every sample once: its best trials equal the brute force over `trial`."""
+    t, x = _noisy(n=300, seed=5)
+    assert np.isnan(ot.trial(t, x, 0.25, 3, 0.3, 4, 6, 0.0204)[0])
+    assert np.isfinite(ot.trial(t, x, 0.35, 3, 0.3, 4, 6, 0.0204)[0])
+    widths = np.array([0.1, 0.3, 0.6])
+    periods = np.array([0.015, 0.2, 0.25, 0.31, 0.7])
+    got = ot.tlsfold(t, x, periods, widths, 6, 0.0204)
+    for p, (sn, w, m, e) in enumerate(_brute(t, x, periods, widths, 6, 0.0204)):
+        assert (got["w"][p], got["l"][p], got["e"][p]) == (w, m, e)
+        if w >= 0:
+            assert widths[w] < periods[p]
+            np.testing.assert_allclose(got["sn"][p], sn, rtol=1e-12)
+
+
+def test_too_few_samples_give_no_trial():
+    for n, masked in ((1, 0), (3, 0), (10, 8), (6, 6)):
+        t = (np.arange(n) * 0.0204).astype(np.float32)
+        x = np.ones(n, np.float32)
+        t[:masked] = -1000.0
+        r = ot.tlsfold(t, x, np.array([0.5, 0.03, 2.0]), np.linspace(0.04, 0.3, 10), 10, 0.0204)
+        assert np.all(np.isneginf(r["sn"])) and np.all(r["w"] == -1), (n, masked, r)
