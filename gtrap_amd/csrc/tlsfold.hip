@@ -315,6 +315,35 @@ __device__ __forceinline__ int tf_cell(double v, double base, double inv, int nc
   return (int)c;
 }
 
+// Lane move of a double by a DPP control (both 32-bit halves).
+template <int CTRL>
+__device__ __forceinline__ double tf_dpp(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Inclusive scan over the 64 lanes of a wave with DPP moves (no LDS
+// crossbar): row_shr 1, 2, 4, 8 inside each row of 16 lanes, then
+// row_bcast:15 into rows 1 and 3 and row_bcast:31 into rows 2 and 3.
+__device__ __forceinline__ double tf_wave_iscan(double v, int lane) {
+  const int rl = lane & 15;
+  double t = tf_dpp<0x111>(v);
+  if (rl >= 1) v += t;
+  t = tf_dpp<0x112>(v);
+  if (rl >= 2) v += t;
+  t = tf_dpp<0x114>(v);
+  if (rl >= 4) v += t;
+  t = tf_dpp<0x118>(v);
+  if (rl >= 8) v += t;
+  t = tf_dpp<0x142>(v);
+  if ((lane & 31) >= 16) v += t;
+  t = tf_dpp<0x143>(v);
+  if (lane >= 32) v += t;
+  return v;
+}
+
 constexpr int kPer = kTfChunk / kTfThreads;
 __device__ void tf_stage(TfLds &L, const TfRec *rec, int nv, double P, int lo,
                          int cnt, double r, double xref) {
@@ -361,13 +390,7 @@ __device__ void tf_stage(TfLds &L, const TfRec *rec, int nv, double P, int lo,
 #pragma unroll
   for (int v = 0; v < 5; ++v) inc[v] = acc[v];
 #pragma unroll
-  for (int o = 1; o < kWave; o <<= 1) {
-#pragma unroll
-    for (int v = 0; v < 5; ++v) {
-      const double y = __shfl_up(inc[v], o);
-      if (lane >= o) inc[v] += y;
-    }
-  }
+  for (int v = 0; v < 5; ++v) inc[v] = tf_wave_iscan(inc[v], lane);
   if (lane == kWave - 1) {
 #pragma unroll
     for (int v = 0; v < 5; ++v) L.scan[wv][v] = inc[v];
