@@ -304,118 +304,120 @@ __device__ __forceinline__ void tf_bnd(double W, const double *wrow, int b, doub
   kind = 0;
 }
 
-// Stage extended range [lo, lo+cnt) (cnt <= kTfChunk) into LDS with
-// exclusive prefix sums of (x', x'^2, q, q^2, x'q), q = image - r.  Thread t
-// owns kPer consecutive elements; the scan is a wave shuffle scan plus a
-// cross-wave fix-up.
 // cell of v in the phase index of a staged range
 __device__ __forceinline__ int tf_cell(double v, double base, double inv, int nc) {
-  double c = (v - base) * inv;
-  c = c < 0.0 ? 0.0 : (c > (double)(nc - 1) ? (double)(nc - 1) : c);
-  return (int)c;
+  return (int)fmin(fmax((v - base) * inv, 0.0), (double)(nc - 1));  // NaN -> 0
 }
 
-// Lane move of a double by a DPP control (both 32-bit halves).
-template <int CTRL>
-__device__ __forceinline__ double tf_dpp(double v) {
+// Lane move of a double by a DPP control (both 32-bit halves).  Lanes whose
+// source lies outside their row, and lanes of rows outside ROWS, read +0.0.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double tf_dpp0(double v) {
   const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xf, 0xf, false);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWS, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xf, true);
   return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
-// Inclusive scan over the 64 lanes of a wave with DPP moves (no LDS
-// crossbar): row_shr 1, 2, 4, 8 inside each row of 16 lanes, then
-// row_bcast:15 into rows 1 and 3 and row_bcast:31 into rows 2 and 3.
-__device__ __forceinline__ double tf_wave_iscan(double v, int lane) {
-  const int rl = lane & 15;
-  double t = tf_dpp<0x111>(v);
-  if (rl >= 1) v += t;
-  t = tf_dpp<0x112>(v);
-  if (rl >= 2) v += t;
-  t = tf_dpp<0x114>(v);
-  if (rl >= 4) v += t;
-  t = tf_dpp<0x118>(v);
-  if (rl >= 8) v += t;
-  t = tf_dpp<0x142>(v);
-  if ((lane & 31) >= 16) v += t;
-  t = tf_dpp<0x143>(v);
-  if (lane >= 32) v += t;
-  return v;
+// Inclusive scans over the 64 lanes of a wave of N values at once, with DPP
+// moves (no LDS crossbar): row_shr 1, 2, 4, 8 inside each row of 16 lanes,
+// then row_bcast:15 into rows 1 and 3 and row_bcast:31 into rows 2 and 3.  A
+// lane without a source adds +0.0, which leaves every sum as it is (but for
+// the sign of a zero).
+template <int CTRL, int ROWS, int N>
+__device__ __forceinline__ void tf_scan_step(double (&v)[N]) {
+  double t[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) t[i] = tf_dpp0<CTRL, ROWS>(v[i]);
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] += t[i];
+}
+template <int N>
+__device__ __forceinline__ void tf_wave_iscan(double (&v)[N]) {
+  tf_scan_step<0x111, 0xf>(v);
+  tf_scan_step<0x112, 0xf>(v);
+  tf_scan_step<0x114, 0xf>(v);
+  tf_scan_step<0x118, 0xf>(v);
+  tf_scan_step<0x142, 0xa>(v);
+  tf_scan_step<0x143, 0xc>(v);
 }
 
+// Stage extended range [lo, lo+cnt) (cnt <= kTfChunk) into LDS: the images
+// lu, the phase index and the exclusive prefix sums of (x', x'^2, q, q^2,
+// x'q), q = image - r.  Thread t owns the kPer consecutive elements from
+// k0 = t kPer and reads them (and the image before them, for the phase index)
+// straight from the records, so one barrier (after the wave totals) and the
+// closing one are all it needs.
 constexpr int kPer = kTfChunk / kTfThreads;
 __device__ void tf_stage(TfLds &L, const TfRec *rec, int nv, double P, int lo,
                          int cnt, double r, double xref) {
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
-  // coalesced loads (element u * kTfThreads + tid, every load in flight before
-  // the first use) into lu and, as x' - xref, into the rows' first prefix slot
+  const int k0 = tid * kPer;
+  // image of extended element j = lo + k of record value u
+  auto image = [&](int k, double u) {
+    const int j = lo + k;
+    return j < 0 ? u - P : (j >= nv ? u + P : u);
+  };
+  double img[kPer], xs[kPer], qs[kPer];
+  double acc[5] = {0, 0, 0, 0, 0};
   if (cnt > 0) {
-    double iv[kPer];
-    float xv[kPer];
+    // every load in flight before the first use: the range ends (the same two
+    // records for every thread), the element before k0 and the own elements
+    const int last = cnt - 1;
+    const double ub = rec[tf_src(nv, lo)].u, ut = rec[tf_src(nv, lo + last)].u;
+    const double up = rec[tf_src(nv, lo + min(max(k0 - 1, 0), last))].u;
+    TfRec rv[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) rv[u] = rec[tf_src(nv, lo + min(k0 + u, last))];
+    const double base = image(0, ub), top = image(last, ut);
+    // phase index: two cells per sample; cells narrower than 1e-9 of the
+    // magnitudes (a range of near-equal phases) fall back to one cell
+    const int nc = 2 * min(cnt, kTfChunk);
+    const double span = top - base;
+    const double inv =
+        span > 1e-9 * (1.0 + fabs(base) + fabs(top)) * (double)nc ? (double)nc / span : 0.0;
+    int cp = k0 == 0 ? -1 : tf_cell(image(k0 - 1, up), base, inv, nc);
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
-      const int js = tf_src(nv, lo + min(u * kTfThreads + tid, cnt - 1));
-      const TfRec rv = rec[js];
-      iv[u] = rv.u;
-      xv[u] = rv.x;
-    }
-#pragma unroll
-    for (int u = 0; u < kPer; ++u) {
-      const int k = u * kTfThreads + tid, j = lo + k;
+      const int k = k0 + u;
+      img[u] = image(k, rv[u].u);
+      xs[u] = 0.0;
+      qs[u] = 0.0;
       if (k < cnt) {
-        L.lu[k] = j < 0 ? iv[u] - P : (j >= nv ? iv[u] + P : iv[u]);
-        L.px[k][0] = (double)xv[u] - xref;
+        L.lu[k] = img[u];
+        xs[u] = (double)rv[u].x - xref;
+        qs[u] = img[u] - r;
+        acc[0] += xs[u]; acc[1] += xs[u] * xs[u]; acc[2] += qs[u]; acc[3] += qs[u] * qs[u];
+        acc[4] += xs[u] * qs[u];
+        // cells (cp, ck] start at k: the first two by predicated stores,
+        // longer runs (gaps in phase) by a loop
+        const int ck = tf_cell(img[u], base, inv, nc);
+        if (ck > cp) L.tab[cp + 1] = (unsigned short)k;
+        if (ck > cp + 1) L.tab[cp + 2] = (unsigned short)k;
+        for (int g = cp + 3; g <= ck; ++g) L.tab[g] = (unsigned short)k;
+        cp = ck;
       }
     }
-  }
-  __syncthreads();
-  double xs[kPer], qs[kPer];
-  double acc[5] = {0, 0, 0, 0, 0};
-#pragma unroll
-  for (int u = 0; u < kPer; ++u) {
-    const int k = tid * kPer + u;
-    xs[u] = 0.0;
-    qs[u] = 0.0;
-    if (k < cnt) {
-      const double img = L.lu[k];
-      xs[u] = L.px[k][0];
-      qs[u] = img - r;
-      acc[0] += xs[u]; acc[1] += xs[u] * xs[u]; acc[2] += qs[u]; acc[3] += qs[u] * qs[u];
-      acc[4] += xs[u] * qs[u];
-    }
+    const int cl = tf_cell(top, base, inv, nc);
+    for (int g = cl + 1 + tid; g <= nc; g += kTfThreads) L.tab[g] = (unsigned short)cnt;
+    if (tid == 0) { L.tbase = base; L.tinv = inv; L.tnc = nc; }
+  } else if (tid == 0) {
+    L.tab[0] = 0;
+    L.tab[1] = 0;
+    L.tbase = 0.0;
+    L.tinv = 0.0;
+    L.tnc = 1;
   }
   // inclusive wave scan of the per-thread totals
   double inc[5];
 #pragma unroll
   for (int v = 0; v < 5; ++v) inc[v] = acc[v];
-#pragma unroll
-  for (int v = 0; v < 5; ++v) inc[v] = tf_wave_iscan(inc[v], lane);
+  tf_wave_iscan(inc);
   if (lane == kWave - 1) {
 #pragma unroll
     for (int v = 0; v < 5; ++v) L.scan[wv][v] = inc[v];
   }
   __syncthreads();
-  {  // phase index: two cells per sample; cells narrower than 1e-9 of the
-     // magnitudes (a range of near-equal phases) fall back to one cell
-    const double base = cnt > 0 ? L.lu[0] : 0.0, top = cnt > 0 ? L.lu[cnt - 1] : 0.0;
-    const int nc = cnt < 1 ? 1 : 2 * (cnt > kTfChunk ? kTfChunk : cnt);
-    const double span = top - base;
-    const double inv =
-        span > 1e-9 * (1.0 + fabs(base) + fabs(top)) * (double)nc ? (double)nc / span : 0.0;
-#pragma unroll
-    for (int u = 0; u < kPer; ++u) {
-      const int k = tid * kPer + u;
-      if (k < cnt) {
-        const int ck = tf_cell(L.lu[k], base, inv, nc);
-        const int cp = k == 0 ? -1 : tf_cell(L.lu[k - 1], base, inv, nc);
-        for (int g = cp + 1; g <= ck; ++g) L.tab[g] = (unsigned short)k;
-      }
-    }
-    const int cl = cnt > 0 ? tf_cell(top, base, inv, nc) : -1;
-    for (int g = cl + 1 + tid; g <= nc; g += kTfThreads) L.tab[g] = (unsigned short)cnt;
-    if (tid == 0) { L.tbase = base; L.tinv = inv; L.tnc = nc; }
-  }
   double run[5];
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
@@ -425,7 +427,7 @@ __device__ void tf_stage(TfLds &L, const TfRec *rec, int nv, double P, int lo,
   }
 #pragma unroll
   for (int u = 0; u < kPer; ++u) {
-    const int k = tid * kPer + u;
+    const int k = k0 + u;
     if (k < cnt) {
 #pragma unroll
       for (int v = 0; v < 5; ++v) L.px[k][v] = run[v];
@@ -437,7 +439,7 @@ __device__ void tf_stage(TfLds &L, const TfRec *rec, int nv, double P, int lo,
       }
     }
   }
-  if (cnt == 0 && tid == 0) {
+  if (cnt <= 0 && tid == 0) {
 #pragma unroll
     for (int v = 0; v < 5; ++v) L.px[0][v] = 0.0;
   }
