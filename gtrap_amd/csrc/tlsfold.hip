@@ -478,15 +478,17 @@ __device__ __forceinline__ TfSums tf_cum(const TfLds &L, int p, double s) {
 // Both cases are index selects (no branch); g = tf_cell(t0 + c) is the
 // clamped floor of the same f.
 constexpr double kTfEdge = 1e-4;
-__device__ __forceinline__ void tf_bracket(const TfLds &L, double t0, double c, int &a, int &b) {
+__device__ __forceinline__ void tf_bracket_f(const TfLds &L, double f, int &a, int &b) {
   const int nc = L.tnc;
-  const double f = (t0 + c - L.tbase) * L.tinv;
   const double fl = floor(f), fr = f - fl;
   const int gi = (int)fmin(fmax(fl, -1.0), (double)nc);  // NaN -> -1
   const bool inner = (unsigned)gi < (unsigned)nc && fabs(fr - 0.5) <= 0.5 - kTfEdge;
   const int g = min(max(gi, 0), nc - 1);
   a = L.tab[inner ? gi : max(g - 1, 0)];
   b = L.tab[inner ? gi + 1 : min(g + 2, nc)];
+}
+__device__ __forceinline__ void tf_bracket(const TfLds &L, double t0, double c, int &a, int &b) {
+  tf_bracket_f(L, (t0 + c - L.tbase) * L.tinv, a, b);
 }
 
 __device__ __forceinline__ int tf_clamp(int r, int lo, int hi) {
@@ -905,8 +907,10 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
                 wlt + __umul24((unsigned)kTfWl, __umul24((unsigned)w, (unsigned)nl) + (unsigned)m);
             const double cb[4] = {bd[0], bd[1], bd[2], bd[3]};
             int ba[4], bb[4], br[4];
+            // the phase-index coordinate of t0 + c is f0 + c tinv
+            const double tinv = L.tinv, f0 = (t0 - L.tbase) * tinv;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) tf_bracket(L, t0, cb[i], ba[i], bb[i]);
+            for (int i = 0; i < 4; ++i) tf_bracket_f(L, fma(cb[i], tinv, f0), ba[i], bb[i]);
             tf_count4(L, ba, bb, t0, cb, br);
             if ((bb[0] - ba[0] > kTfCount) | (bb[1] - ba[1] > kTfCount) |
                 (bb[2] - ba[2] > kTfCount) | (bb[3] - ba[3] > kTfCount)) {
