@@ -278,12 +278,14 @@ constexpr int kTfUnroll = 4;          // global loads in flight per thread in th
 // Static LDS of a workgroup.  The per-(W, boundary) accumulators of the chunked
 // sweep and the per-(epoch, W) window table are sized at launch (dynamic LDS).
 struct TfLds {
-  double lu[kTfChunk + 8];      // staged images (exact); 8 pad entries for tf_count4
+  // staged images (exact), then 8 entries of +inf (read by the bisections)
+  double lu[kTfChunk + 8];
   double px[kTfChunk + 1][5];   // exclusive prefix of x', x'^2, q, q^2, x'q
-  // phase index of the staged range (two cells per sample): tab[g] = first k
+  // phase index of the staged range (two cells per sample): T(g) = first k
   // with cell(lu[k]) >= g, cell(v) = clamp(floor((v - tbase) * tinv), 0, tnc - 1),
-  // tab[tnc] = cnt
-  unsigned short tab[kTfCells + 1];
+  // for g = -1 .. tnc + 1 (T(-1) = 0, T(tnc) = T(tnc + 1) = cnt), stored as
+  // tab[g + 1]
+  unsigned short tab[kTfCells + 3];
   double tbase, tinv;
   int tnc;
   double scan[kTfThreads / kWave][5];
@@ -392,22 +394,22 @@ __device__ void tf_stage(TfLds &L, const TfRec *rec, int nv, double P, int lo,
         // cells (cp, ck] start at k: the first two by predicated stores,
         // longer runs (gaps in phase) by a loop
         const int ck = tf_cell(img[u], base, inv, nc);
-        if (ck > cp) L.tab[cp + 1] = (unsigned short)k;
-        if (ck > cp + 1) L.tab[cp + 2] = (unsigned short)k;
-        for (int g = cp + 3; g <= ck; ++g) L.tab[g] = (unsigned short)k;
+        if (ck > cp) L.tab[cp + 2] = (unsigned short)k;
+        if (ck > cp + 1) L.tab[cp + 3] = (unsigned short)k;
+        for (int g = cp + 3; g <= ck; ++g) L.tab[g + 1] = (unsigned short)k;
         cp = ck;
       }
     }
     const int cl = tf_cell(top, base, inv, nc);
-    for (int g = cl + 1 + tid; g <= nc; g += kTfThreads) L.tab[g] = (unsigned short)cnt;
-    if (tid == 0) { L.tbase = base; L.tinv = inv; L.tnc = nc; }
+    for (int g = cl + 1 + tid; g <= nc + 1; g += kTfThreads) L.tab[g + 1] = (unsigned short)cnt;
+    if (tid == 0) { L.tab[0] = 0; L.tbase = base; L.tinv = inv; L.tnc = nc; }
   } else if (tid == 0) {
-    L.tab[0] = 0;
-    L.tab[1] = 0;
+    for (int g = 0; g < 4; ++g) L.tab[g] = 0;
     L.tbase = 0.0;
     L.tinv = 0.0;
     L.tnc = 1;
   }
+  if (tid < 8) L.lu[max(cnt, 0) + tid] = INFINITY;
   // inclusive wave scan of the per-thread totals
   double inc[5];
 #pragma unroll
@@ -471,8 +473,8 @@ __device__ __forceinline__ TfSums tf_cum(const TfLds &L, int p, double s) {
 // magnitudes wide (tf_stage), so the rounding of f and of the offset test is
 // below 1e-4 of a cell: when f lies at least 1e-4 of a cell inside cell g,
 // every sample of a lower cell fails the test and every sample of a higher
-// cell passes it, and the answer lies in [tab[g], tab[g+1]].  Otherwise the
-// answer lies in [tab[g-1], tab[g+2]] (one whole cell of margin each side).
+// cell passes it, and the answer lies in [T(g), T(g+1)].  Otherwise the
+// answer lies in [T(g-1), T(g+2)] (one whole cell of margin each side).
 // The result is clamped to [lo, hi], which equals the search restricted to
 // [lo, hi) (the test is monotone in the sorted images).
 // Both cases are index selects (no branch); g = tf_cell(t0 + c) is the
@@ -483,9 +485,9 @@ __device__ __forceinline__ void tf_bracket_f(const TfLds &L, double f, int &a, i
   const double fl = floor(f), fr = f - fl;
   const int gi = (int)fmin(fmax(fl, -1.0), (double)nc);  // NaN -> -1
   const bool inner = (unsigned)gi < (unsigned)nc && fabs(fr - 0.5) <= 0.5 - kTfEdge;
-  const int g = min(max(gi, 0), nc - 1);
-  a = L.tab[inner ? gi : max(g - 1, 0)];
-  b = L.tab[inner ? gi + 1 : min(g + 2, nc)];
+  const int g = min(max(gi, 0), nc - 1), in = inner ? 1 : 0;  // inner: g = gi
+  a = L.tab[g + in];      // T(g - 1 + in)
+  b = L.tab[g + 3 - in];  // T(g + 2 - in)
 }
 __device__ __forceinline__ void tf_bracket(const TfLds &L, double t0, double c, int &a, int &b) {
   tf_bracket_f(L, (t0 + c - L.tbase) * L.tinv, a, b);
@@ -499,11 +501,12 @@ __device__ __forceinline__ int tf_clamp(int r, int lo, int hi) {
 // brackets [a[i], b[i]) of at most kTfCount samples, by a branch-free
 // bisection of fixed depth (steps 4, 2, 1) with the four searches in step:
 // [a[i], r[i]) holds only failing samples throughout, and a step advances
-// r[i] past samples inside the bracket only.  Reads reach at most index
-// a[i] + 6 <= kTfChunk + 6, inside lu's padding; values read past b[i] are
-// masked.
+// r[i] past failing samples only.  Samples at or past b[i] lie in later
+// cells than the target (or are the +inf padding) and pass, so reads past the
+// bracket need no mask.  Reads reach at most index a[i] + 6 <= cnt + 6,
+// inside lu's padding.
 constexpr int kTfCount = 7;
-__device__ __forceinline__ void tf_count4(const TfLds &L, const int *a, const int *b, double t0,
+__device__ __forceinline__ void tf_count4(const TfLds &L, const int *a, double t0,
                                           const double *c, int *r) {
 #pragma unroll
   for (int i = 0; i < 4; ++i) r[i] = a[i];
@@ -517,8 +520,7 @@ __device__ __forceinline__ void tf_count4(const TfLds &L, const int *a, const in
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int k = r[i] + st - 1;
-      const int fail = (int)(k < b[i]) & (int)!(d[i] >= c[i]);
+      const int fail = (int)!(d[i] >= c[i]);
       r[i] += fail * st;
     }
   }
@@ -526,14 +528,12 @@ __device__ __forceinline__ void tf_count4(const TfLds &L, const int *a, const in
 
 // One boundary of either kind (0: d >= c, 1: d > c) in a bracket of at most
 // kTfCount samples, by the same fixed-depth bisection as tf_count4.
-__device__ __forceinline__ int tf_bisect(const TfLds &L, int a, int b, double t0, double c,
-                                         int kind) {
+__device__ __forceinline__ int tf_bisect(const TfLds &L, int a, double t0, double c, int kind) {
   int r = a;
 #pragma unroll
   for (int st = 4; st > 0; st >>= 1) {
-    const int k = r + st - 1;
-    const double d = L.lu[k] - t0;
-    const int fail = (int)(k < b) & (int)!(kind ? d > c : d >= c);
+    const double d = L.lu[r + st - 1] - t0;
+    const int fail = (int)!(kind ? d > c : d >= c);
     r += fail * st;
   }
   return r;
@@ -859,9 +859,9 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
             int a0, b0, a1, b1;
             tf_bracket(L, t0, -0.5 * W, a0, b0);
             tf_bracket(L, t0, 0.5 * W, a1, b1);
-            int p0 = tf_bisect(L, a0, b0, t0, -0.5 * W, 0);
-            int pE = tf_bisect(L, a1, b1, t0, 0.5 * W, 0);
-            int pM = tf_bisect(L, a1, b1, t0, 0.5 * W, 1);
+            int p0 = tf_bisect(L, a0, t0, -0.5 * W, 0);
+            int pE = tf_bisect(L, a1, t0, 0.5 * W, 0);
+            int pM = tf_bisect(L, a1, t0, 0.5 * W, 1);
             if ((b0 - a0 > kTfCount) | (b1 - a1 > kTfCount)) {
               if (b0 - a0 > kTfCount) p0 = tf_search(L.lu, a0, b0, t0, -0.5 * W, 0);
               if (b1 - a1 > kTfCount) {
@@ -911,7 +911,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
             const double tinv = L.tinv, f0 = (t0 - L.tbase) * tinv;
 #pragma unroll
             for (int i = 0; i < 4; ++i) tf_bracket_f(L, fma(cb[i], tinv, f0), ba[i], bb[i]);
-            tf_count4(L, ba, bb, t0, cb, br);
+            tf_count4(L, ba, t0, cb, br);
             if ((bb[0] - ba[0] > kTfCount) | (bb[1] - ba[1] > kTfCount) |
                 (bb[2] - ba[2] > kTfCount) | (bb[3] - ba[3] > kTfCount)) {
 #pragma unroll
@@ -968,7 +968,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
               const double t0 = jt0[j], cval = jc[j];
               int a, b;
               tf_bracket(L, t0, cval, a, b);
-              int p = tf_bisect(L, a, b, t0, cval, jkind[j]);
+              int p = tf_bisect(L, a, t0, cval, jkind[j]);
               if (b - a > kTfCount) p = tf_search(L.lu, a, b, t0, cval, jkind[j]);
               p = tf_clamp(p, 0, cc);
               const TfSums cs = tf_cum(L, p, t0 - r);
