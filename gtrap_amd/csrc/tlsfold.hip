@@ -283,9 +283,9 @@ struct TfLds {
   double px[kTfChunk + 1][5];   // exclusive prefix of x', x'^2, q, q^2, x'q
   // phase index of the staged range (two cells per sample): T(g) = first k
   // with cell(lu[k]) >= g, cell(v) = clamp(floor((v - tbase) * tinv), 0, tnc - 1),
-  // for g = -1 .. tnc + 1 (T(-1) = 0, T(tnc) = T(tnc + 1) = cnt), stored as
-  // tab[g + 1]
-  unsigned short tab[kTfCells + 3];
+  // for g = -2 .. tnc + 2 (T(-2) = T(-1) = 0, T(tnc) .. T(tnc + 2) = cnt),
+  // stored as tab[g + 2]
+  unsigned short tab[kTfCells + 5];
   double tbase, tinv;
   int tnc;
   double scan[kTfThreads / kWave][5];
@@ -394,17 +394,17 @@ __device__ void tf_stage(TfLds &L, const TfRec *rec, int nv, double P, int lo,
         // cells (cp, ck] start at k: the first two by predicated stores,
         // longer runs (gaps in phase) by a loop
         const int ck = tf_cell(img[u], base, inv, nc);
-        if (ck > cp) L.tab[cp + 2] = (unsigned short)k;
-        if (ck > cp + 1) L.tab[cp + 3] = (unsigned short)k;
-        for (int g = cp + 3; g <= ck; ++g) L.tab[g + 1] = (unsigned short)k;
+        if (ck > cp) L.tab[cp + 3] = (unsigned short)k;
+        if (ck > cp + 1) L.tab[cp + 4] = (unsigned short)k;
+        for (int g = cp + 3; g <= ck; ++g) L.tab[g + 2] = (unsigned short)k;
         cp = ck;
       }
     }
     const int cl = tf_cell(top, base, inv, nc);
-    for (int g = cl + 1 + tid; g <= nc + 1; g += kTfThreads) L.tab[g + 1] = (unsigned short)cnt;
-    if (tid == 0) { L.tab[0] = 0; L.tbase = base; L.tinv = inv; L.tnc = nc; }
+    for (int g = cl + 1 + tid; g <= nc + 2; g += kTfThreads) L.tab[g + 2] = (unsigned short)cnt;
+    if (tid == 0) { L.tab[0] = 0; L.tab[1] = 0; L.tbase = base; L.tinv = inv; L.tnc = nc; }
   } else if (tid == 0) {
-    for (int g = 0; g < 4; ++g) L.tab[g] = 0;
+    for (int g = 0; g < 6; ++g) L.tab[g] = 0;
     L.tbase = 0.0;
     L.tinv = 0.0;
     L.tnc = 1;
@@ -477,25 +477,27 @@ __device__ __forceinline__ TfSums tf_cum(const TfLds &L, int p, double s) {
 // answer lies in [T(g-1), T(g+2)] (one whole cell of margin each side).
 // The result is clamped to [lo, hi], which equals the search restricted to
 // [lo, hi) (the test is monotone in the sorted images).
-// Both cases are index selects (no branch); g = tf_cell(t0 + c) is the
-// clamped floor of the same f.
+// Both cases are index selects (no branch).  gi, the floor of f clamped to
+// [-1, tnc] (tf_cell(t0 + c) for gi inside [0, tnc)), is never inner outside
+// [0, tnc); there [T(gi-1), T(gi+2)] still holds the answer (every sample
+// passes for gi = -1, fails up to cell tnc - 1 for gi = tnc), and T's padding
+// covers gi - 1 >= -2 and gi + 2 <= tnc + 2.
 constexpr double kTfEdge = 1e-4;
 __device__ __forceinline__ void tf_bracket_f(const TfLds &L, double f, int &a, int &b) {
   const int nc = L.tnc;
   const double fl = floor(f), fr = f - fl;
   const int gi = (int)fmin(fmax(fl, -1.0), (double)nc);  // NaN -> -1
   const bool inner = (unsigned)gi < (unsigned)nc && fabs(fr - 0.5) <= 0.5 - kTfEdge;
-  const int g = min(max(gi, 0), nc - 1), in = inner ? 1 : 0;  // inner: g = gi
-  a = L.tab[g + in];      // T(g - 1 + in)
-  b = L.tab[g + 3 - in];  // T(g + 2 - in)
+  const int in = inner ? 1 : 0;
+  a = L.tab[gi + 1 + in];  // T(gi - 1 + in)
+  b = L.tab[gi + 4 - in];  // T(gi + 2 - in)
 }
 __device__ __forceinline__ void tf_bracket(const TfLds &L, double t0, double c, int &a, int &b) {
   tf_bracket_f(L, (t0 + c - L.tbase) * L.tinv, a, b);
 }
 
-__device__ __forceinline__ int tf_clamp(int r, int lo, int hi) {
-  return r < lo ? lo : (r > hi ? hi : r);
-}
+// r clamped to [lo, hi] (lo <= hi)
+__device__ __forceinline__ int tf_clamp(int r, int lo, int hi) { return min(max(r, lo), hi); }
 
 // First index passing the offset test (kind 0, d >= c) in each of four
 // brackets [a[i], b[i]) of at most kTfCount samples, by a branch-free
