@@ -477,17 +477,21 @@ __device__ __forceinline__ TfSums tf_cum(const TfLds &L, int p, double s) {
 // answer lies in [T(g-1), T(g+2)] (one whole cell of margin each side).
 // The result is clamped to [lo, hi], which equals the search restricted to
 // [lo, hi) (the test is monotone in the sorted images).
-// Both cases are index selects (no branch).  gi, the floor of f clamped to
-// [-1, tnc] (tf_cell(t0 + c) for gi inside [0, tnc)), is never inner outside
-// [0, tnc); there [T(gi-1), T(gi+2)] still holds the answer (every sample
-// passes for gi = -1, fails up to cell tnc - 1 for gi = tnc), and T's padding
-// covers gi - 1 >= -2 and gi + 2 <= tnc + 2.
+// Both cases are index selects (no branch).  gi is the floor of f clamped to
+// [-1, tnc] (tf_cell(t0 + c) for gi inside [0, tnc)).  Outside, both
+// brackets still hold the answer: f below 0 (gi = -1) lies below every
+// sample's cell coordinate, so every sample passes and the answer is 0 =
+// T(-1) = T(0); f at or above tnc (gi = tnc) lies at or above every sample's
+// (the top sample's is tnc), so with the margin every sample fails and the
+// answer is cnt = T(tnc) = T(tnc + 1); near 0 or tnc the wide bracket
+// [T(gi-1), T(gi+2)] covers both outcomes.  T's padding covers gi - 1 >= -2
+// and gi + 2 <= tnc + 2.
 constexpr double kTfEdge = 1e-4;
 __device__ __forceinline__ void tf_bracket_f(const TfLds &L, double f, int &a, int &b) {
   const int nc = L.tnc;
   const double fl = floor(f), fr = f - fl;
   const int gi = (int)fmin(fmax(fl, -1.0), (double)nc);  // NaN -> -1
-  const bool inner = (unsigned)gi < (unsigned)nc && fabs(fr - 0.5) <= 0.5 - kTfEdge;
+  const bool inner = fabs(fr - 0.5) <= 0.5 - kTfEdge;
   const int in = inner ? 1 : 0;
   a = L.tab[gi + 1 + in];  // T(gi - 1 + in)
   b = L.tab[gi + 4 - in];  // T(gi + 2 - in)
@@ -889,13 +893,13 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
           int mm = nlt ? tid % nlt : 0, w = nlt ? (tid / nlt) % nw : 0,
               je = nlt ? tid / (nlt * nw) : 0;
           for (int k = tid; k < ntr; k += kTfThreads) {
-            if (k != tid) {
+            if (k != tid) {  // wrap by an unsigned min: m - n wraps above m when m < n
               mm += dm;
               const int c1 = mm >= nlt ? 1 : 0;
-              mm -= c1 * nlt;
+              mm = (int)min((unsigned)mm, (unsigned)(mm - nlt));
               w += dw + c1;
               const int c2 = w >= nw ? 1 : 0;
-              w -= c2 * nw;
+              w = (int)min((unsigned)w, (unsigned)(w - nw));
               je += dje + c2;
             }
             // row offsets by 24-bit multiplies (all operands < 2^24)
@@ -914,8 +918,10 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) tf_bracket_f(L, fma(cb[i], tinv, f0), ba[i], bb[i]);
             tf_count4(L, ba, t0, cb, br);
-            if ((bb[0] - ba[0] > kTfCount) | (bb[1] - ba[1] > kTfCount) |
-                (bb[2] - ba[2] > kTfCount) | (bb[3] - ba[3] > kTfCount)) {
+            // bracket widths are >= 0: one of them exceeds kTfCount = 7 iff
+            // their bitwise or does
+            if (((bb[0] - ba[0]) | (bb[1] - ba[1]) | (bb[2] - ba[2]) | (bb[3] - ba[3])) >
+                kTfCount) {
 #pragma unroll
               for (int i = 0; i < 4; ++i)
                 if (bb[i] - ba[i] > kTfCount) br[i] = tf_search(L.lu, ba[i], bb[i], t0, cb[i], 0);
