@@ -62,17 +62,20 @@ __device__ __forceinline__ int tf_xkey(float x) {
 // A trial's rank key is k = sign(S/N) * (S/N)^2, a monotone function of the
 // S/N, so trials compare by key; the S/N and H of a thread's best trial come
 // from its key and its numerator and denominator of H (tf_sn, tf_h).
+// A trial's (w, l, e) is packed as ((w nl + l) << 32 | e) + 1, whose integer
+// order is the (w, l, e) order; 0 means "none yet".
 struct TfBest {
   double key, numB, denA;
-  int w, l, e;
+  unsigned long long idx;
 };
+__device__ __forceinline__ unsigned long long tf_idx(int w, int l, int e, int nl) {
+  return ((unsigned long long)(unsigned)(w * nl + l) << 32 | (unsigned)e) + 1ull;
+}
 
-// better: larger key, ties to the earliest (w, l, e); w < 0 means "none yet"
-__device__ __forceinline__ bool tf_better(double key, int w, int l, int e, const TfBest &b) {
-  if (key > b.key) return true;
-  if (key == b.key && b.w >= 0)
-    return (w < b.w) || (w == b.w && (l < b.l || (l == b.l && e < b.e)));
-  return false;
+// better: larger key, ties to the earliest (w, l, e).  Against "none yet"
+// (key -inf, idx 0) every key but NaN and -inf is better, as in the oracle.
+__device__ __forceinline__ bool tf_better(double key, unsigned long long idx, const TfBest &b) {
+  return key > b.key || (key == b.key && idx < b.idx);
 }
 
 // Region boundary offsets of width W and step m (tls.py:175-179):
@@ -197,16 +200,17 @@ __device__ __forceinline__ double tf_eval(const TfRegions &g, double W, const do
   *numBout = numB;
   *denAout = denA;
   constexpr double kNaN = __builtin_nan("");
-  // n - 3 degrees of freedom: fewer than 4 samples is no trial (the direct
-  // formula gives 0/0 there); denA = 0 makes H infinite and the S/N NaN
-  if (nn <= 3.0 || denA == 0.0) return kNaN;
   const double s2 = numB * numB * (nn - 3.0);
   const bool neg = (numB < 0.0) != (denA < 0.0);
+  const double k = s2 / R;  // every case below is a select (no branch)
+  double key = neg ? -k : k;
   // the expanded residual of a perfect fit can round below zero: a residual
   // of 0 gives an infinite S/N (0/0, NaN, when H = 0 too)
-  if (R <= 0.0) return numB == 0.0 ? kNaN : (neg ? -INFINITY : INFINITY);
-  const double k = s2 / R;
-  return neg ? -k : k;
+  if (R <= 0.0) key = numB == 0.0 ? kNaN : (neg ? -INFINITY : INFINITY);
+  // n - 3 degrees of freedom: fewer than 4 samples is no trial (the direct
+  // formula gives 0/0 there); denA = 0 makes H infinite and the S/N NaN
+  if (nn <= 3.0 || denA == 0.0) key = kNaN;
+  return key;
 }
 
 // S/N and H of a best trial from its key and H's numerator and denominator
@@ -290,7 +294,7 @@ struct TfLds {
   int tnc;
   double scan[kTfThreads / kWave][5];
   double red_key[kTfThreads / kWave], red_nb[kTfThreads / kWave], red_da[kTfThreads / kWave];
-  int red_w[kTfThreads / kWave], red_l[kTfThreads / kWave], red_e[kTfThreads / kWave];
+  unsigned long long red_idx[kTfThreads / kWave];
 };
 
 // boundary b of width W (kind, offset) from the width's row of the (W, L)
@@ -832,7 +836,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
     }
 
     // ---- 2/3. epoch tiles ---------------------------------------------
-    TfBest best{-INFINITY, NAN, NAN, -1, -1, -1};
+    TfBest best{-INFINITY, NAN, NAN, 0ull};
     // first guess of the tile size from the mean phase density
     const double per_epoch = (double)nv / P * A.dtep;
     int te = (int)(((double)kTfChunk / fmax(per_epoch, 1e-9) - 2.0 * (hwm + marg) / A.dtep) - 3.0);
@@ -893,15 +897,6 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
           int mm = nlt ? tid % nlt : 0, w = nlt ? (tid / nlt) % nw : 0,
               je = nlt ? tid / (nlt * nw) : 0;
           for (int k = tid; k < ntr; k += kTfThreads) {
-            if (k != tid) {  // wrap by an unsigned min: m - n wraps above m when m < n
-              mm += dm;
-              const int c1 = mm >= nlt ? 1 : 0;
-              mm = (int)min((unsigned)mm, (unsigned)(mm - nlt));
-              w += dw + c1;
-              const int c2 = w >= nw ? 1 : 0;
-              w = (int)min((unsigned)w, (unsigned)(w - nw));
-              je += dje + c2;
-            }
             // row offsets by 24-bit multiplies (all operands < 2^24)
             const unsigned j = __umul24((unsigned)je, (unsigned)nw) + (unsigned)w;
             const int ep = eb + je, m = mm + 1;
@@ -933,7 +928,16 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
             double nB, dA;
             const double key =
                 tf_stat_px(L, p0, pAB, pBC, pCD, pDE, pE, wd[0], wd[1], s, W, bd, &nB, &dA);
-            if (tf_better(key, w, m, ep, best)) best = TfBest{key, nB, dA, w, m, ep};
+            const unsigned long long idx = tf_idx(w, m, ep, nl);
+            if (tf_better(key, idx, best)) best = TfBest{key, nB, dA, idx};
+            // next trial (wrap by an unsigned min: m - n wraps above m when m < n)
+            mm += dm;
+            const int c1 = mm >= nlt ? 1 : 0;
+            mm = (int)min((unsigned)mm, (unsigned)(mm - nlt));
+            w += dw + c1;
+            const int c2 = w >= nw ? 1 : 0;
+            w = (int)min((unsigned)w, (unsigned)(w - nw));
+            je += dje + c2;
           }
           __syncthreads();
         }
@@ -1010,7 +1014,8 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
                               get(6 + 4 * mm), get(1), get(2), W, wlt + kTfWl * (w * nl + m),
                               &nB, &dA)
                     : __builtin_nan("");  // W >= P: no trial
-          if (tf_better(key, w, m, e0 + el, best)) best = TfBest{key, nB, dA, w, m, e0 + el};
+          const unsigned long long idx = tf_idx(w, m, e0 + el, nl);
+          if (tf_better(key, idx, best)) best = TfBest{key, nB, dA, idx};
         }
         __syncthreads();
         e0 += tce;
@@ -1024,28 +1029,27 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
     for (int o = kWave / 2; o > 0; o >>= 1) {
       const double ok = __shfl_xor(best.key, o), onb = __shfl_xor(best.numB, o),
                    oda = __shfl_xor(best.denA, o);
-      const int ow = __shfl_xor(best.w, o), ol = __shfl_xor(best.l, o),
-                oe = __shfl_xor(best.e, o);
-      if (ow >= 0 && (best.w < 0 || tf_better(ok, ow, ol, oe, best)))
-        best = TfBest{ok, onb, oda, ow, ol, oe};
+      const unsigned long long oidx = __shfl_xor(best.idx, o);
+      if (oidx != 0ull && tf_better(ok, oidx, best)) best = TfBest{ok, onb, oda, oidx};
     }
     if ((tid & (kWave - 1)) == 0) {
       const int wv = tid / kWave;
       L.red_key[wv] = best.key; L.red_nb[wv] = best.numB; L.red_da[wv] = best.denA;
-      L.red_w[wv] = best.w; L.red_l[wv] = best.l; L.red_e[wv] = best.e;
+      L.red_idx[wv] = best.idx;
     }
     __syncthreads();
     if (tid == 0) {
-      TfBest b{L.red_key[0], L.red_nb[0], L.red_da[0], L.red_w[0], L.red_l[0], L.red_e[0]};
+      TfBest b{L.red_key[0], L.red_nb[0], L.red_da[0], L.red_idx[0]};
       for (int k = 1; k < kTfThreads / kWave; ++k)
-        if (L.red_w[k] >= 0 &&
-            (b.w < 0 || tf_better(L.red_key[k], L.red_w[k], L.red_l[k], L.red_e[k], b)))
-          b = TfBest{L.red_key[k], L.red_nb[k], L.red_da[k], L.red_w[k], L.red_l[k], L.red_e[k]};
+        if (L.red_idx[k] != 0ull && tf_better(L.red_key[k], L.red_idx[k], b))
+          b = TfBest{L.red_key[k], L.red_nb[k], L.red_da[k], L.red_idx[k]};
+      const bool found = b.idx != 0ull;
+      const int wl = found ? (int)((b.idx - 1ull) >> 32) : -1;
       A.sn[item] = tf_sn(b.key);
-      if (A.hbest) A.hbest[item] = b.w >= 0 ? tf_h(b.numB, b.denA) : NAN;
-      A.bw[item] = b.w;
-      A.bl[item] = b.l;
-      A.be[item] = b.e;
+      if (A.hbest) A.hbest[item] = found ? tf_h(b.numB, b.denA) : NAN;
+      A.bw[item] = found ? wl / nl : -1;
+      A.bl[item] = found ? wl % nl : -1;
+      A.be[item] = found ? (int)((b.idx - 1ull) & 0xffffffffull) : -1;
     }
     __syncthreads();
   }
