@@ -542,12 +542,18 @@ __device__ __forceinline__ void tf_count4(const TfLds &L, const int *a, double t
   }
 }
 
-// One boundary of either kind (0: d >= c, 1: d > c) in a bracket of at most
-// kTfCount samples, by the same fixed-depth bisection as tf_count4.
-__device__ __forceinline__ int tf_bisect(const TfLds &L, int a, double t0, double c, int kind) {
+// One boundary of either kind (0: d >= c, 1: d > c) in a bracket [a, b) of
+// at most kTfCount samples, by the same fixed-depth bisection as tf_count4
+// (its step of 4 only for a bracket of more than 3 samples).
+__device__ __forceinline__ int tf_bisect(const TfLds &L, int a, int b, double t0, double c,
+                                         int kind) {
   int r = a;
+  if (b - a > 3) {
+    const double d = L.lu[a + 3] - t0;
+    r += (int)!(kind ? d > c : d >= c) * 4;
+  }
 #pragma unroll
-  for (int st = 4; st > 0; st >>= 1) {
+  for (int st = 2; st > 0; st >>= 1) {
     const double d = L.lu[r + st - 1] - t0;
     const int fail = (int)!(kind ? d > c : d >= c);
     r += fail * st;
@@ -875,9 +881,9 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
             int a0, b0, a1, b1;
             tf_bracket(L, t0, -0.5 * W, a0, b0);
             tf_bracket(L, t0, 0.5 * W, a1, b1);
-            int p0 = tf_bisect(L, a0, t0, -0.5 * W, 0);
-            int pE = tf_bisect(L, a1, t0, 0.5 * W, 0);
-            int pM = tf_bisect(L, a1, t0, 0.5 * W, 1);
+            int p0 = tf_bisect(L, a0, b0, t0, -0.5 * W, 0);
+            int pE = tf_bisect(L, a1, b1, t0, 0.5 * W, 0);
+            int pM = tf_bisect(L, a1, b1, t0, 0.5 * W, 1);
             if ((b0 - a0 > kTfCount) | (b1 - a1 > kTfCount)) {
               if (b0 - a0 > kTfCount) p0 = tf_search(L.lu, a0, b0, t0, -0.5 * W, 0);
               if (b1 - a1 > kTfCount) {
@@ -986,7 +992,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
               const double t0 = jt0[j], cval = jc[j];
               int a, b;
               tf_bracket(L, t0, cval, a, b);
-              int p = tf_bisect(L, a, t0, cval, jkind[j]);
+              int p = tf_bisect(L, a, b, t0, cval, jkind[j]);
               if (b - a > kTfCount) p = tf_search(L.lu, a, b, t0, cval, jkind[j]);
               p = tf_clamp(p, 0, cc);
               const TfSums cs = tf_cum(L, p, t0 - r);
