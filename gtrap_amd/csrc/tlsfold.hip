@@ -678,6 +678,8 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
     // are kept in the workspace for the tile phase.
     // n/2 buckets (counted over all samples, so the counting pass also finds
     // the number of valid samples and the first valid index)
+    // (each block's workspace slot holds nmax/2 + 2 bucket ends after the
+    // records, gtrap_tlsfold_ws_bytes: more buckets need a larger slot)
     const int nb = max(1, min(n / 2, kTfMaxBuckets));
     const double bscale = (double)nb / P;
     const double invP = 1.0 / P;
