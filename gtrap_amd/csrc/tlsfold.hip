@@ -808,7 +808,8 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
             for (int j = a1; j < a2; ++j) {
               const double ju = cu[j];
               const int jk = tf_xkey(cx[j]);
-              rank += (ju < ku || (ju == ku && (jk < kk || (jk == kk && j < k)))) ? 1 : 0;
+              // bitwise, not short-circuit: no branch per comparison
+              rank += (int)((ju < ku) | ((ju == ku) & ((jk < kk) | ((jk == kk) & (j < k)))));
             }
             const int dst = s0 + min(a1 + rank, cnt - 1);
             rec[dst] = TfRec{ku, kx, 0};

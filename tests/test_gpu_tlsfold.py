@@ -112,3 +112,17 @@ def test_longest_serving_curve(gpu):
     short period, a middle one and a long one (few cycles per window)."""
     t, x = _curve(70000, 23, mask=0.02)
     _check([(t, x)], [np.array([0.6, 23.3, 466.0])], np.linspace(0.04, 0.3, 10), 10)
+
+
+def test_gapped_curve(gpu):
+    """Two 13-day sectors with a 1-day mid-sector gap each and 10 days between
+    them: at periods near and above the gap lengths whole stretches of phase
+    hold no sample, so the phase index has long runs of empty cells and many
+    boundaries fall in them or past the staged range's ends."""
+    rng = np.random.default_rng(31)
+    t = np.arange(0.0, 36.0, 0.0204)
+    keep = ((t < 6.0) | ((t >= 7.0) & (t < 13.0)) | ((t >= 23.0) & (t < 29.0)) | (t >= 30.0))
+    t = t[keep]
+    x = 1.0 + 1e-3 * rng.standard_normal(t.size)
+    _check([(t.astype(np.float32), x.astype(np.float32))],
+           [np.array([0.9, 1.3, 2.7, 6.6, 11.0, 17.5, 40.0])], np.linspace(0.04, 0.3, 10), 10)
