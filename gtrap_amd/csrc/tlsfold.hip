@@ -271,8 +271,8 @@ constexpr int kTfMaxTe = 48;     // epochs per window-table run (at most)
 constexpr int kTfWl = 10;  // (W, L) table row: tAB, tBC, tCD, tDE, 4L^2, 8L^2, 4WL, 16L,
                            // 2/L, 1/L^2
 constexpr int kTfCells = 2 * kTfChunk;  // phase index cells of a staged range (at most)
-constexpr int kTfMaxTile = 96;   // epochs per staged tile: with the reference at the
-                                 // tile centre, |q| stays within 48 epoch steps + W/2
+constexpr int kTfMaxTile = 144;  // epochs per staged tile: with the reference at the
+                                 // tile centre, |q| stays within 72 epoch steps + W/2
 constexpr int kTfMaxT = 4;    // chunked sweep: (epoch, W, boundary) sums per thread
 constexpr int kTfMaxBuckets = 30000;  // phase buckets (LDS counters, <= 120 KB)
 constexpr int kTfSortChunk = 7680;    // samples per in-LDS bucket-sort chunk (90 KB + ends)
