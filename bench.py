@@ -31,7 +31,7 @@ TRIAL_FLOPS = 103
 # measured offline: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch of
 # tlsfold_kernel<true>, scaled by the same counters over a 1 GiB copy
 # (factors 2.0 and 1.0)
-SERVE_FOLD_TRAFFIC = 1.321e11
+SERVE_FOLD_TRAFFIC = 1.317e11
 
 
 def log(msg: str) -> None:
