@@ -277,7 +277,7 @@ constexpr int kTfMaxT = 4;    // chunked sweep: (epoch, W, boundary) sums per th
 constexpr int kTfMaxBuckets = 30000;  // phase buckets (LDS counters, <= 120 KB)
 constexpr int kTfSortChunk = 7680;    // samples per in-LDS bucket-sort chunk (90 KB + ends)
 constexpr int kTfMaxBreaks = 64;      // sort chunks per period
-constexpr int kTfUnroll = 4;          // global loads in flight per thread in the sort loops
+constexpr int kTfUnroll = 8;          // global loads in flight per thread in the sort loops
 
 // Static LDS of a workgroup.  The per-(W, boundary) accumulators of the chunked
 // sweep and the per-(epoch, W) window table are sized at launch (dynamic LDS).
