@@ -515,8 +515,9 @@ __device__ __forceinline__ int tf_clamp(int r, int lo, int hi) { return min(max(
 // cells than the target (or are the +inf padding) and pass, so reads past the
 // bracket need no mask.  Reads reach at most index a[i] + 6 <= cnt + 6,
 // inside lu's padding.  The step of 4 tests sample a[i] + 3, which passes
-// when every bracket holds at most 3 samples (wide = 0): it is then skipped,
-// and a wave whose lanes all skip it saves a quarter of the search.
+// (leaving r[i] unchanged) when bracket i holds at most 3 samples; when all
+// four do (wide = 0) the step is skipped, and a wave whose lanes all skip it
+// saves a quarter of the search.
 constexpr int kTfCount = 7;
 __device__ __forceinline__ void tf_count4(const TfLds &L, const int *a, double t0,
                                           const double *c, bool wide, int *r) {
