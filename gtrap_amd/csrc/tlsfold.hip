@@ -1,15 +1,15 @@
 // Periodic trapezoid least squares (tls_fold) for gfx950.
 //
-// Statistic: gtrap/tls.py:173-208 (get_chi2Hmax, divx=True, sigma=1) with
-// S/N = -H/sqrt(res/(n-3)) (tls.py:229), evaluated on time folded at each
+// Statistic: gtrap/tls.py:44-79 (get_chi2Hmax, divx=True, sigma=1) with
+// S/N = -H/sqrt(res/(n-3)) (tls.py:100), evaluated on time folded at each
 // trial period.  Definition (identical in the oracle):
 //   u = fmod(t, P) (float64, exact), epoch e < ceil(P/dtep) at t0 = e*dtep,
 //   d = u - t0, or (u - P) - t0 if d >= P/2, or (u + P) - t0 if d < -P/2;
-//   L = m * (0.45*W/nl); regions A..E of tls.py:175-179 on d; mean over
+//   L = m * (0.45*W/nl); regions A..E of tls.py:46-50 on d; mean over
 //   [-W/2, W/2]; a trial needs n > 3 samples in [-W/2, W/2) (n - 3 degrees
 //   of freedom) and W < P (a window as long as the period would hold two
 //   images of a sample), else it is NaN; best trial per period = first
-//   maximum in (W, L, epoch) order with NaN skipped (tlsfit, tls.py:217-231).
+//   maximum in (W, L, epoch) order with NaN skipped (tlsfit, tls.py:82-110).
 //
 // MI355X design.  One 512-thread workgroup owns one (curve, period) work item
 // at a time (persistent grid, atomic ticket, per-block global workspace):
@@ -59,6 +59,12 @@ __device__ __forceinline__ int tf_xkey(float x) {
   return b ^ ((b >> 31) & 0x7fffffff);
 }
 
+// A sample takes part when its time is valid (t >= 0; masked samples carry a
+// negative time) and its flux is finite; other samples are skipped.
+__device__ __forceinline__ bool tf_usable(float t, float x) {
+  return t >= 0.0f && isfinite(x);
+}
+
 // A trial's rank key is k = sign(S/N) * (S/N)^2, a monotone function of the
 // S/N, so trials compare by key; the S/N and H of a thread's best trial come
 // from its key and its numerator and denominator of H (tf_sn, tf_h).
@@ -78,7 +84,7 @@ __device__ __forceinline__ bool tf_better(double key, unsigned long long idx, co
   return key > b.key || (key == b.key && idx < b.idx);
 }
 
-// Region boundary offsets of width W and step m (tls.py:175-179):
+// Region boundary offsets of width W and step m (tls.py:46-50):
 // kind 0: first index with d >= c ; kind 1: first index with d > c.
 __device__ __forceinline__ void tf_bounds(double W, int m, int nl, double &tAB, double &tBC,
                                           double &tCD, double &tDE) {
@@ -98,6 +104,40 @@ struct __align__(16) TfRec {
   float x;
   int pad;
 };
+
+// (u, flux bits) order of two records
+__device__ __forceinline__ bool tf_rec_less(const TfRec &a, const TfRec &b) {
+  return a.u < b.u || (a.u == b.u && tf_xkey(a.x) < tf_xkey(b.x));
+}
+
+// Sorts rec[0, m) by (u, flux bits) with the whole workgroup: a bitonic network
+// in its all-ascending form (every merge opens with a flip step), over the
+// next power of two N >= m with virtual +inf elements at m..N-1, so a
+// compare-exchange whose partner lies past m leaves its element in place.
+// O(m log^2 m) compare-exchanges on the workspace; it serves phase buckets
+// too large for one LDS chunk (strongly clustered phases, e.g. repeated
+// time stamps) and curves with more LDS chunks than the chunk table holds.
+__device__ void tf_bitonic(TfRec *rec, int m) {
+  int N = 1;
+  while (N < m) N <<= 1;
+  for (int k = 2; k <= N; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int p = threadIdx.x; p < (N >> 1); p += kTfThreads) {
+        // pair p -> its lower element i (a zero bit inserted at j's position)
+        const int i = ((p & ~(j - 1)) << 1) | (p & (j - 1));
+        const int q = j == (k >> 1) ? (i ^ (k - 1)) : (i + j);
+        if (q < m) {
+          const TfRec ri = rec[i], rq = rec[q];
+          if (tf_rec_less(rq, ri)) {
+            rec[i] = rq;
+            rec[q] = ri;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
 
 // Extended index j of the sorted folded curve (nv samples) maps to sample
 // tf_src(nv, j); its image is u - P for j < 0, u + P for j >= nv, else u.
@@ -159,7 +199,7 @@ struct TfRegions {
 };
 
 // Window statistics of an (epoch, W): mean over the closed window and the
-// sum of squares of x' - mean over [-W/2, W/2) (tls.py:187 and the residual's
+// sum of squares of x' - mean over [-W/2, W/2) (tls.py:57-59 and the residual's
 // constant term).
 __device__ __forceinline__ void tf_window(double mcnt, double sM, double nAll, double s1All,
                                           double s2All, double &mean, double &y2) {
@@ -276,7 +316,7 @@ constexpr int kTfMaxTile = 144;  // epochs per staged tile: with the reference a
 constexpr int kTfMaxT = 4;    // chunked sweep: (epoch, W, boundary) sums per thread
 constexpr int kTfMaxBuckets = 30000;  // phase buckets (LDS counters, <= 120 KB)
 constexpr int kTfSortChunk = 7680;    // samples per in-LDS bucket-sort chunk (90 KB + ends)
-constexpr int kTfMaxBreaks = 64;      // sort chunks per period
+constexpr int kTfMaxBreaks = 128;     // sort chunks per period (longer curves: one bitonic sort)
 constexpr int kTfUnroll = 8;          // global loads in flight per thread in the sort loops
 
 // Static LDS of a workgroup.  The per-(W, boundary) accumulators of the chunked
@@ -671,6 +711,19 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
     const float *xc = A.x + A.off[c];
     const int n = A.len[c];
     const int nE = (int)ceil(P / A.dtep);
+    if (n > A.nmax) {
+      // a curve longer than the workspace slot is not searched (the host
+      // entry points guard this where the lengths are known): S/N NaN and
+      // indices -2 mark the item
+      if (tid == 0) {
+        A.sn[item] = NAN;
+        if (A.hbest) A.hbest[item] = NAN;
+        A.bw[item] = -2;
+        A.bl[item] = -2;
+        A.be[item] = -2;
+      }
+      continue;
+    }
 
     // ---- 1. fold and bucket-sort by phase ------------------------------
     // Counting sort into ~2-sample phase buckets with LDS counters (the tile
@@ -691,13 +744,17 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
     {
       int myv = 0, myi0 = n;
       for (int i0 = tid; i0 < n; i0 += kTfUnroll * kTfThreads) {
-        float tv[kTfUnroll];  // every load in flight before the first use
+        float tv[kTfUnroll], xv[kTfUnroll];  // every load in flight before the first use
 #pragma unroll
-        for (int v = 0; v < kTfUnroll; ++v) tv[v] = tc[min(i0 + v * kTfThreads, n - 1)];
+        for (int v = 0; v < kTfUnroll; ++v) {
+          const int i = min(i0 + v * kTfThreads, n - 1);
+          tv[v] = tc[i];
+          xv[v] = xc[i];
+        }
 #pragma unroll
         for (int v = 0; v < kTfUnroll; ++v) {
           const int i = i0 + v * kTfThreads;
-          if (i >= n || !(tv[v] >= 0.0f)) continue;
+          if (i >= n || !tf_usable(tv[v], xv[v])) continue;
           ++myv;
           myi0 = min(myi0, i);
           const double u = tf_fmod((double)tv[v], P, invP);
@@ -709,7 +766,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
     }
     __syncthreads();
     const int nv = s_cnt;
-    // x_ref = flux of the first valid sample (any constant cancels in the statistic)
+    // x_ref = flux of the first usable sample (any constant cancels in the statistic)
     const double xref = (s_i0 < n) ? (double)xc[s_i0] : 0.0;
     {  // exclusive scan of lcur[0..nb) in place (per-thread runs + wave scan)
       const int per = (nb + kTfThreads - 1) / kTfThreads;
@@ -741,7 +798,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
       }
 #pragma unroll
       for (int v = 0; v < kTfUnroll; ++v) {
-        if (i0 + v * kTfThreads >= n || !(tv[v] >= 0.0f)) continue;
+        if (i0 + v * kTfThreads >= n || !tf_usable(tv[v], xv[v])) continue;
         const double u = tf_fmod((double)tv[v], P, invP);
         const int pos = atomicAdd(&lcur[min(nb - 1, (int)(u * bscale))], 1);
         rec[pos] = TfRec{u, xv[v], 0};
@@ -755,7 +812,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
       s_brk[nbrk++] = 0;
       while (bb < nb && nbrk < kTfMaxBreaks) {
         const int s0 = bb == 0 ? 0 : lcur[bb - 1];
-        int lo_b = bb + 1, hi_b = nb;
+        int lo_b = bb + 1, hi_b = min(nb, bb + kTfSortChunk);
         while (lo_b < hi_b) {
           const int mid = (lo_b + hi_b + 1) >> 1;
           if (lcur[mid - 1] - s0 <= kTfSortChunk) lo_b = mid; else hi_b = mid - 1;
@@ -763,7 +820,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
         bb = lo_b;
         s_brk[nbrk++] = bb;
       }
-      s_nbrk = (bb >= nb) ? nbrk : -1;  // -1: too many chunks, sort in place
+      s_nbrk = (bb >= nb) ? nbrk : -1;  // -1: more chunks than the table holds
     }
     for (int b = tid; b <= nb; b += kTfThreads) cursor[b] = lcur[b];
     __syncthreads();
@@ -817,38 +874,16 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
           }
           __syncthreads();
         } else {
-          // one bucket larger than a chunk (strongly clustered phases)
-          for (int b = bb + tid; b < be; b += kTfThreads) {
-            const int a1 = b == 0 ? 0 : cursor[b - 1], a2 = cursor[b];
-            for (int k = a1 + 1; k < a2; ++k) {
-              const TfRec kr = rec[k];
-              int j = k - 1;
-              while (j >= a1 && (rec[j].u > kr.u ||
-                                 (rec[j].u == kr.u && tf_xkey(rec[j].x) > tf_xkey(kr.x)))) {
-                rec[j + 1] = rec[j];
-                --j;
-              }
-              rec[j + 1] = kr;
-            }
-          }
-          __syncthreads();
+          // one bucket larger than a chunk (strongly clustered phases; a
+          // chunk of several buckets holds at most kTfSortChunk samples)
+          tf_bitonic(rec + s0, cnt);
         }
       }
     } else {
-      for (int b = tid; b < nb; b += kTfThreads) {
-        const int a1 = b == 0 ? 0 : cursor[b - 1], a2 = cursor[b];
-        for (int k = a1 + 1; k < a2; ++k) {
-          const TfRec kr = rec[k];
-          int j = k - 1;
-          while (j >= a1 && (rec[j].u > kr.u ||
-                             (rec[j].u == kr.u && tf_xkey(rec[j].x) > tf_xkey(kr.x)))) {
-            rec[j + 1] = rec[j];
-            --j;
-          }
-          rec[j + 1] = kr;
-        }
-      }
-      __syncthreads();
+      // more chunks than the chunk table holds (a very long curve): one
+      // bitonic sort of the whole folded curve orders every bucket too,
+      // since the bucket index is non-decreasing in u
+      tf_bitonic(rec, nv);
     }
 
     // ---- 2/3. epoch tiles ---------------------------------------------
@@ -1076,11 +1111,12 @@ __global__ void tf_zero_ticket(int *p) { *p = 0; }
 // Dynamic LDS of a launch: the (W, L) table and widths, then te_run epochs
 // of window table (2 doubles + 3 ints per (epoch, W)), te_run as large as
 // the LDS left over by the static part allows (at most kTfMaxTe).
+template <bool kItems>
 static int tf_launch_cfg(int nw, int nl, size_t *dyn, int *te_run) {
-  static size_t static_lds = 0;
+  static size_t static_lds = 0;  // per instantiation
   if (!static_lds) {
     hipFuncAttributes fa;
-    if (hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(tlsfold_kernel<false>)) !=
+    if (hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(tlsfold_kernel<kItems>)) !=
         hipSuccess) {
       set_error("tlsfold: hipFuncGetAttributes failed");
       return GTRAP_ELAUNCH;
@@ -1136,7 +1172,7 @@ extern "C" int gtrap_tlsfold(double *snr, double *hbest, int32_t *best_w, int32_
   if (nptot == 0) return GTRAP_OK;
   size_t dyn = 0;
   int te_run = 0;
-  if (const int rc = gtrap::tf_launch_cfg(nw, nl, &dyn, &te_run)) return rc;
+  if (const int rc = gtrap::tf_launch_cfg<false>(nw, nl, &dyn, &te_run)) return rc;
   gtrap::TfArgs A{t, x, off, len, periods, poff, nullptr, ncurves, nptot, widths, nw, nl, dtep,
                   snr, hbest, best_w, best_l, best_e, (unsigned char *)ws,
                   tf_slot_bytes(nmax), nmax, nullptr, nullptr, nullptr, nullptr, te_run};
@@ -1168,7 +1204,7 @@ extern "C" int gtrap_tlsfold_items(double *snr, double *hbest, int32_t *best_w, 
                 "gtrap_tlsfold_items: workspace too small");
   size_t dyn = 0;
   int te_run = 0;
-  if (const int rc = gtrap::tf_launch_cfg(nw, nl, &dyn, &te_run)) return rc;
+  if (const int rc = gtrap::tf_launch_cfg<true>(nw, nl, &dyn, &te_run)) return rc;
   gtrap::TfArgs A{t, x, off, len, periods, nullptr, nullptr, 0, 0, widths, nw, nl, dtep,
                   snr, hbest, best_w, best_l, best_e, (unsigned char *)ws,
                   tf_slot_bytes(nmax), nmax, item_curve, item_pidx, nitems_dev, counter, te_run};
@@ -1180,9 +1216,10 @@ extern "C" int gtrap_tlsfold_items(double *snr, double *hbest, int32_t *best_w, 
 namespace gtrap {
 
 // Periodogram statistics of one curve's periods [poff[c], poff[c]+np[c]):
-// best = first maximum of S/N (NaN and -inf skipped), mean and standard
-// deviation over the finite S/N values, SDE = (best - mean)/std (the BLS
-// statistic of gtrap/getstat.py:114-118 applied to the TLS periodogram).
+// best = first maximum of S/N over the values that are neither NaN nor -inf
+// (+inf, a perfect fit, can win), mean and standard deviation over the finite
+// S/N values, SDE = (best - mean)/std (the BLS statistic of
+// gtrap/getstat.py:114-118 applied to the TLS periodogram).
 __global__ __launch_bounds__(256) void tlsstats_kernel(const double *__restrict__ sn,
                                                        const int64_t *__restrict__ poff,
                                                        const int32_t *__restrict__ np_,
@@ -1198,11 +1235,12 @@ __global__ __launch_bounds__(256) void tlsstats_kernel(const double *__restrict_
   int64_t bi = -1;
   for (int64_t i = threadIdx.x; i < m; i += blockDim.x) {
     const double v = sn[a + i];
+    // a perfect fit (+inf) may win; -inf marks a period without any trial
+    if (v > -INFINITY && (bi < 0 || v > bs)) { bs = v; bi = i; }
     if (isfinite(v)) {
       s1 += v;
       s2 += v * v;
       cnt += 1.0;
-      if (v > bs) { bs = v; bi = i; }
     }
   }
   __shared__ double r_bs[256], r_s1[256], r_s2[256], r_c[256];

@@ -21,11 +21,26 @@ Two paths compute exactly the same per-request result:
   items ran in this step.  The step is a fixed-shape sequence of kernels whose
   variable parts (segment list, item list, counts) live in device descriptor
   buffers, so it is captured once as HIP graphs (detrend, fold, statistics)
-  and replayed.  Requests that arrive while a step runs join the next one:
-  their work overlaps with the work of requests admitted earlier.
+  and replayed.
+
+Steps are pipelined: while step k runs on the device, the host admits the
+requests that arrived, stages their data through pinned buffers and uploads
+it on a side stream, and plans step k+1; the results of step k are gathered
+on the device and copied to pinned host memory behind its kernels, so
+collecting them waits for step k only.  Requests that arrive while a step runs
+join the next one: their work overlaps with the work of requests admitted
+earlier.
 
 Each work item's computation depends only on its own curve and period, so a
 result is the same alone and in any mix (bit for bit).
+
+Failure handling: a request that fails validation is refused by
+:meth:`TransitServer.submit`; a request whose upload fails is retired alone
+with an error result; a step whose launch fails retires the requests that had
+work items in it (error results) and the server keeps serving the others;
+:meth:`TransitServer.cancel` removes a queued or resident request between
+steps.  The device side is a backend object (:class:`HipBackend`), so the
+host logic runs on the CPU against a test backend.
 """
 from __future__ import annotations
 
@@ -66,10 +81,21 @@ class Result:
     h: float
     arrival: float = 0.0
     done: float = 0.0
+    error: str | None = None
+    periodogram: dict | None = None   # per-period sn, h, w, l, e (ServerConfig.periodogram)
 
     @property
     def latency(self) -> float:
         return self.done - self.arrival
+
+    @property
+    def ok(self) -> bool:
+        return self.error is None
+
+
+def _failed(req: Request, msg: str, now: float) -> Result:
+    return Result(req.rid, float("nan"), -1, float("nan"), float("nan"), -1, -1, -1,
+                  float("nan"), req.arrival, now, error=msg)
 
 
 class FirstFit:
@@ -106,6 +132,26 @@ class FirstFit:
         return self.cap - sum(n for _, n in self.free)
 
 
+def validate(req: Request, max_len: int | None = None) -> None:
+    """Raise ValueError for a request the search cannot take: no periods, a
+    non-positive or non-finite period, mismatched or empty arrays, non-finite
+    flux (the median detrend assumes finite values), or more than max_len
+    samples."""
+    n = len(req.t)
+    if n == 0 or len(req.flux) != n:
+        raise ValueError(f"request {req.rid}: t and flux must be non-empty and of equal length")
+    if max_len is not None and n > max_len:
+        raise ValueError(f"request {req.rid}: {n} samples > max_len {max_len}")
+    per = np.asarray(req.periods, np.float64)
+    if per.size == 0:
+        raise ValueError(f"request {req.rid}: empty period grid")
+    if not np.all(np.isfinite(per) & (per > 0)):
+        raise ValueError(f"request {req.rid}: periods must be finite and positive")
+    if not np.all(np.isfinite(np.asarray(req.flux, np.float32))):
+        raise ValueError(f"request {req.rid}: flux must be finite (mask samples by a "
+                         "negative time instead)")
+
+
 # ----------------------------------------------------------------------------
 # plain path: one launch sequence per curve
 # ----------------------------------------------------------------------------
@@ -118,11 +164,12 @@ class PlainSearch:
 
 
 def search_plain(req: Request, widths, nl=10, dtep=0.0204, r=64, mode="div", nblocks=512,
-                 ctx: PlainSearch | None = None) -> Result:
+                 ctx: PlainSearch | None = None, periodogram: bool = False) -> Result:
     import torch
     from . import gfilter
     from .tlsfold import FoldBatch, tlsfold_device
     _native.require_gpu()
+    validate(req)
     ctx = ctx or PlainSearch()
     n, m = len(req.t), len(req.periods)
     t = torch.from_numpy(np.ascontiguousarray(req.t, np.float32)).cuda()
@@ -149,6 +196,8 @@ def search_plain(req: Request, widths, nl=10, dtep=0.0204, r=64, mode="div", nbl
                  float(sde.item()), int(out["w"][k]) if k >= 0 else -1,
                  int(out["l"][k]) if k >= 0 else -1, int(out["e"][k]) if k >= 0 else -1,
                  float(out["h"][k]) if k >= 0 else float("nan"), req.arrival)
+    if periodogram:
+        res.periodogram = {key: v.cpu().numpy() for key, v in out.items()}
     return res
 
 
@@ -179,6 +228,7 @@ class ServerConfig:
     nblocks: int = 1024              # persistent tls_fold workgroups
     seg_len: int = 512               # detrend rows per lane
     use_graph: bool = True
+    periodogram: bool = False        # return every request's per-period results
 
     @classmethod
     def tuned(cls, **kw) -> "ServerConfig":
@@ -192,17 +242,46 @@ class ServerConfig:
         return cfg
 
 
-class TransitServer:
-    def __init__(self, cfg: ServerConfig | None = None):
+@dataclass
+class StepPlan:
+    """Host description of one device step."""
+    new: list            # slots admitted since the previous step (detrend)
+    seg_curve: np.ndarray
+    seg_start: np.ndarray
+    item_curve: np.ndarray
+    item_pidx: np.ndarray
+    done: list           # slots whose last items run in this step (statistics)
+    slot_off: dict       # slot -> (sample offset, length, period offset, period count)
+    trials: int = 0
+
+
+class _Pinned:
+    """Grow-only pinned host buffer of one dtype."""
+
+    def __init__(self, dtype):
+        self.dtype = dtype
+        self.buf = None
+
+    def get(self, n: int):
+        import torch
+        if self.buf is None or self.buf.numel() < n:
+            self.buf = torch.empty(max(n, 1 << 12), dtype=self.dtype, pin_memory=True)
+        return self.buf[:n]
+
+
+class HipBackend:
+    """Device side of a :class:`TransitServer`: arenas, descriptors, the three
+    captured step graphs, pinned staging for uploads (side stream) and for the
+    results of a step (gathered on the device behind the step)."""
+
+    def __init__(self, cfg: ServerConfig):
         import torch
         _native.require_gpu()
-        self.cfg = cfg = cfg or ServerConfig()
+        self.cfg = cfg
         dev = "cuda"
         pbytes = int(cfg.hbm_budget * cfg.period_share)
         self.scap = max(cfg.max_len, (cfg.hbm_budget - pbytes) // SAMPLE_BYTES)
         self.pcap = max(1, pbytes // PERIOD_BYTES)
-        self.salloc = FirstFit(self.scap)
-        self.palloc = FirstFit(self.pcap)
         f32, f64, i32, i64 = torch.float32, torch.float64, torch.int32, torch.int64
         self.t = torch.zeros(self.scap, dtype=f32, device=dev)
         self.x = torch.zeros(self.scap, dtype=f32, device=dev)
@@ -215,7 +294,7 @@ class TransitServer:
         self.be = torch.zeros(self.pcap, dtype=i32, device=dev)
         C = cfg.max_curves
         self.max_segs = C * ((cfg.max_len + cfg.seg_len - 1) // cfg.seg_len)
-        # device descriptors (fixed addresses, captured by the graph)
+        # device descriptors (fixed addresses, captured by the graphs)
         self.d_off = torch.zeros(C, dtype=i64, device=dev)
         self.d_len = torch.zeros(C, dtype=i32, device=dev)
         self.d_poff = torch.zeros(C, dtype=i64, device=dev)
@@ -229,57 +308,79 @@ class TransitServer:
         self.d_best = torch.zeros(C, dtype=i64, device=dev)
         self.d_bsn = torch.zeros(C, dtype=f64, device=dev)
         self.d_sde = torch.zeros(C, dtype=f64, device=dev)
+        self.d_done = torch.zeros(C, dtype=i64, device=dev)
         self.widths = torch.from_numpy(np.ascontiguousarray(cfg.widths, np.float64)).to(dev)
         self.ws = Workspace()
         self.ws_buf, self.ws_bytes = self.ws.get(cfg.max_len, cfg.nblocks)
-        # pinned host mirrors of the per-step descriptors
+        # two sets of pinned host descriptors: a set is rewritten only after
+        # the step that read it has completed
         pin = dict(pin_memory=True)
-        self.h_off = torch.zeros(C, dtype=i64, **pin)
-        self.h_len = torch.zeros(C, dtype=i32, **pin)
-        self.h_poff = torch.zeros(C, dtype=i64, **pin)
-        self.h_np = torch.zeros(C, dtype=i32, **pin)
-        self.h_seg_curve = torch.zeros(self.max_segs, dtype=i32, **pin)
-        self.h_seg_start = torch.zeros(self.max_segs, dtype=i32, **pin)
-        self.h_item_curve = torch.zeros(cfg.items_per_step, dtype=i32, **pin)
-        self.h_item_pidx = torch.zeros(cfg.items_per_step, dtype=i64, **pin)
-        self.h_stat_curves = torch.zeros(C, dtype=i32, **pin)
-        self.h_counts = torch.zeros(4, dtype=i32, **pin)
-        self.free_slots = list(range(C - 1, -1, -1))
-        self.slots: dict[int, _Slot] = {}
-        self.order: list[int] = []   # resident slots, oldest first
-        self.queue: list[Request] = []
+        self.hdesc = [dict(off=torch.zeros(C, dtype=i64, **pin), len=torch.zeros(C, dtype=i32, **pin),
+                       poff=torch.zeros(C, dtype=i64, **pin), np=torch.zeros(C, dtype=i32, **pin),
+                       seg_curve=torch.zeros(self.max_segs, dtype=i32, **pin),
+                       seg_start=torch.zeros(self.max_segs, dtype=i32, **pin),
+                       item_curve=torch.zeros(cfg.items_per_step, dtype=i32, **pin),
+                       item_pidx=torch.zeros(cfg.items_per_step, dtype=i64, **pin),
+                       stat=torch.zeros(C, dtype=i32, **pin), done=torch.zeros(C, dtype=i64, **pin),
+                       counts=torch.zeros(4, dtype=i32, **pin))
+                  for _ in range(2)]
+        self.slot_tab = {}            # slot -> (soff, n, poff, m), mirrored into both sets
+        self.stage = [(_Pinned(f32), _Pinned(f64)) for _ in range(2)]
+        self.stage_ev = [None, None]  # upload of a staging set finished
+        self.res_host = [_Pinned(f64) for _ in range(2)]
+        self.pg_host = [[] for _ in range(2)]
         self.graphs = None
-        self.fold_log = None  # list of (start event, end event, trials) when set
-        self.steps = 0
+        self.fold_log = None          # list of (start event, end event, trials) when set
         self.stream = torch.cuda.current_stream()
+        self.side = torch.cuda.Stream()   # default priority
+        self.nsteps = 0
+        self._pending_uploads = []    # (slot, soff, poff, req) staged for the next step
+        self.side.wait_stream(self.stream)  # arenas are zeroed on the serving stream
 
-    # -- admission -----------------------------------------------------------
-    def submit(self, req: Request) -> None:
-        if len(req.t) > self.cfg.max_len:
-            raise ValueError(f"request {req.rid}: {len(req.t)} samples > max_len")
-        if len(req.t) > self.scap or len(req.periods) > self.pcap:
-            raise ValueError(f"request {req.rid} exceeds the HBM budget")
-        self.queue.append(req)
+    # -- uploads ---------------------------------------------------------------
+    def upload(self, slot: int, soff: int, poff: int, req: Request) -> None:
+        """Stage a newly admitted request; the copies are issued with the next
+        step (pinned staging, side stream)."""
+        self._pending_uploads.append((slot, soff, poff, req))
+        self.slot_tab[slot] = (soff, len(req.t), poff, len(req.periods))
 
-    def _admit(self, now: float):
-        new = []
-        while self.queue and self.free_slots and self.queue[0].arrival <= now:
-            req = self.queue[0]
-            so = self.salloc.alloc(len(req.t))
-            if so is None:
-                break
-            po = self.palloc.alloc(len(req.periods))
-            if po is None:
-                self.salloc.release(so, len(req.t))
-                break
-            self.queue.pop(0)
-            s = self.free_slots.pop()
-            self.slots[s] = _Slot(req, so, po, 0, now)
-            self.order.append(s)
-            new.append(s)
-        return new
+    def release(self, slot: int) -> None:
+        self.slot_tab.pop(slot, None)
 
-    # -- one step ------------------------------------------------------------
+    def _issue_uploads(self, k: int):
+        import torch
+        ups = self._pending_uploads
+        self._pending_uploads = []
+        if not ups:
+            return None
+        if self.stage_ev[k] is not None:
+            self.stage_ev[k].synchronize()   # staging set k free again
+        nf = sum(2 * len(r.t) for _, _, _, r in ups)
+        nd = sum(len(r.periods) for _, _, _, r in ups)
+        sf = self.stage[k][0].get(nf).numpy()
+        sd = self.stage[k][1].get(nd).numpy()
+        a = b = 0
+        spans = []
+        for slot, soff, poff, r in ups:
+            n, m = len(r.t), len(r.periods)
+            sf[a:a + n] = r.t
+            sf[a + n:a + 2 * n] = r.flux
+            sd[b:b + m] = r.periods
+            spans.append((soff, poff, n, m, a, b))
+            a += 2 * n
+            b += m
+        tf, td = self.stage[k][0].buf, self.stage[k][1].buf
+        with torch.cuda.stream(self.side):
+            for soff, poff, n, m, a, b in spans:
+                self.t[soff:soff + n].copy_(tf[a:a + n], non_blocking=True)
+                self.x[soff:soff + n].copy_(tf[a + n:a + 2 * n], non_blocking=True)
+                self.per[poff:poff + m].copy_(td[b:b + m], non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(self.side)
+        self.stage_ev[k] = ev
+        return ev
+
+    # -- one step --------------------------------------------------------------
     def _rec_pre(self):
         """Ticket reset and detrend of the newly admitted curves."""
         cfg = self.cfg
@@ -310,10 +411,73 @@ class TransitServer:
                      p(self.d_stat_curves), p(self.d_counts[2:3]), cfg.max_curves,
                      p(self.d_best), p(self.d_bsn), p(self.d_sde), _native.stream_handle())
 
-    def _launch(self, trials: int):
-        """One step's fixed kernel sequence: three graphs (detrend, fold,
-        statistics) captured on the first step and replayed after it.  With
-        ``fold_log`` set, HIP events on the serving stream bracket the fold."""
+    def launch(self, plan: StepPlan):
+        """Issue one step (asynchronous): descriptors, the three graphs, the
+        device gather of the finished requests' results into pinned memory.
+        Returns a handle for :meth:`collect`.  At most two steps may be in
+        flight (the pinned descriptor, staging and result sets alternate)."""
+        import torch
+        k = self.nsteps & 1
+        up = self._issue_uploads(k)
+        H = self.hdesc[k]
+        for s, (soff, n, poff, m) in plan.slot_off.items():
+            H["off"][s], H["len"][s], H["poff"][s], H["np"][s] = soff, n, poff, m
+        ns, ni, nd = len(plan.seg_curve), len(plan.item_curve), len(plan.done)
+        if ns:
+            H["seg_curve"][:ns] = torch.from_numpy(plan.seg_curve)
+            H["seg_start"][:ns] = torch.from_numpy(plan.seg_start)
+        if ni:
+            H["item_curve"][:ni] = torch.from_numpy(plan.item_curve)
+            H["item_pidx"][:ni] = torch.from_numpy(plan.item_pidx)
+        for j, s in enumerate(plan.done):
+            H["stat"][j] = s
+            H["done"][j] = s
+        H["counts"][0], H["counts"][1], H["counts"][2] = ns, ni, nd
+        st = self.stream
+        with torch.cuda.stream(st):
+            if up is not None:
+                st.wait_event(up)
+            for d, h in ((self.d_off, H["off"]), (self.d_len, H["len"]),
+                         (self.d_poff, H["poff"]), (self.d_np, H["np"])):
+                d.copy_(h, non_blocking=True)
+            if ns:
+                self.d_seg_curve[:ns].copy_(H["seg_curve"][:ns], non_blocking=True)
+                self.d_seg_start[:ns].copy_(H["seg_start"][:ns], non_blocking=True)
+            if ni:
+                self.d_item_curve[:ni].copy_(H["item_curve"][:ni], non_blocking=True)
+                self.d_item_pidx[:ni].copy_(H["item_pidx"][:ni], non_blocking=True)
+            if nd:
+                self.d_stat_curves[:nd].copy_(H["stat"][:nd], non_blocking=True)
+                self.d_done[:nd].copy_(H["done"][:nd], non_blocking=True)
+            self.d_counts[:3].copy_(H["counts"][:3], non_blocking=True)
+            self._run_graphs(plan.trials)
+            spans = []
+            if nd:
+                # gather best, S/N, SDE and the best trial's w, l, e, H per request
+                idx = self.d_done[:nd]
+                best = self.d_best[idx]
+                bi = best.clamp(min=0)
+                g = torch.stack([best.to(torch.float64), self.d_bsn[idx], self.d_sde[idx],
+                                 self.bw[bi].to(torch.float64), self.bl[bi].to(torch.float64),
+                                 self.be[bi].to(torch.float64), self.h[bi]])
+                hr = self.res_host[k].get(g.numel())
+                hr.copy_(g.reshape(-1), non_blocking=True)
+                if self.cfg.periodogram:
+                    for s in plan.done:
+                        soff, n, poff, m = plan.slot_off[s]
+                        pg = {}
+                        for name, arr in (("sn", self.sn), ("h", self.h), ("w", self.bw),
+                                          ("l", self.bl), ("e", self.be)):
+                            hp = torch.empty(m, dtype=arr.dtype, pin_memory=True)
+                            hp.copy_(arr[poff:poff + m], non_blocking=True)
+                            pg[name] = hp
+                        spans.append(pg)
+            ev = torch.cuda.Event()
+            ev.record(st)
+        self.nsteps += 1
+        return (k, list(plan.done), ev, spans)
+
+    def _run_graphs(self, trials: int):
         import torch
         parts = (self._rec_pre, self._rec_fold, self._rec_post)
         if self.cfg.use_graph and self.graphs is None:
@@ -322,7 +486,7 @@ class TransitServer:
             gs = []
             for f in parts:
                 g = torch.cuda.CUDAGraph()
-                with torch.cuda.graph(g):
+                with torch.cuda.graph(g):  # captured on torch's side stream
                     f()
                 gs.append(g)
             self.graphs = gs
@@ -342,6 +506,23 @@ class TransitServer:
             run[1]()
         run[2]()
 
+    def collect(self, handle):
+        """Wait for a launched step; returns {slot: (best index, S/N, SDE,
+        w, l, e, H, periodogram or None)} of its finished requests."""
+        k, done, ev, spans = handle
+        ev.synchronize()
+        nd = len(done)
+        out = {}
+        if nd:
+            v = self.res_host[k].buf[:7 * nd].numpy().reshape(7, nd).copy()
+            for j, s in enumerate(done):
+                pg = None
+                if spans:
+                    pg = {key: t.numpy().copy() for key, t in spans[j].items()}
+                out[s] = (int(v[0, j]), float(v[1, j]), float(v[2, j]), int(v[3, j]),
+                          int(v[4, j]), int(v[5, j]), float(v[6, j]), pg)
+        return out
+
     def fold_stats(self):
         """(launches, mean fold kernel ms, total trials, total fold seconds) of
         the logged steps; synchronises."""
@@ -351,36 +532,113 @@ class TransitServer:
         tr = sum(k for _, _, k in self.fold_log)
         return len(ms), (float(np.mean(ms)) if ms else 0.0), tr, sum(ms) * 1e-3
 
-    def step(self, now: float | None = None) -> list[Result]:
-        """Admit arrived requests, run one device step, return finished results."""
-        import torch
+
+class TransitServer:
+    def __init__(self, cfg: ServerConfig | None = None, backend=None):
+        self.cfg = cfg = cfg or ServerConfig()
+        self.dev = backend if backend is not None else HipBackend(cfg)
+        self.scap, self.pcap = self.dev.scap, self.dev.pcap
+        self.salloc = FirstFit(self.scap)
+        self.palloc = FirstFit(self.pcap)
+        self.free_slots = list(range(cfg.max_curves - 1, -1, -1))
+        self.slots: dict[int, _Slot] = {}
+        self.order: list[int] = []   # resident slots, oldest first
+        self.queue: list[Request] = []
+        self.steps = 0
+        self.inflight: list = []     # launched, not collected: (handle, slots used, done slots)
+        self.dead: set = set()       # failed or cancelled slots awaiting release
+        self.cancelled: set = set()
+        self._out: list[Result] = []
+
+    # -- compatibility accessors ---------------------------------------------
+    @property
+    def fold_log(self):
+        return self.dev.fold_log
+
+    @fold_log.setter
+    def fold_log(self, v):
+        self.dev.fold_log = v
+
+    def fold_stats(self):
+        return self.dev.fold_stats()
+
+    # -- admission -----------------------------------------------------------
+    def submit(self, req: Request) -> None:
+        validate(req, self.cfg.max_len)
+        if len(req.t) > self.scap or len(req.periods) > self.pcap:
+            raise ValueError(f"request {req.rid} exceeds the HBM budget")
+        self.queue.append(req)
+
+    def cancel(self, rid: int) -> bool:
+        """Remove a queued or resident request; returns whether it was found.
+        Its memory is released once no step in flight uses it; no result is
+        returned for it."""
+        for i, r in enumerate(self.queue):
+            if r.rid == rid:
+                self.queue.pop(i)
+                return True
+        for s, sl in self.slots.items():
+            if sl.req.rid == rid and s not in self.dead:
+                self.cancelled.add(s)
+                self._kill(s)
+                return True
+        return False
+
+    def _in_flight(self, s: int) -> bool:
+        return any(s in used for _, used, _ in self.inflight)
+
+    def _kill(self, s: int) -> None:
+        """Stop scheduling slot s; release it now or when its steps complete."""
+        self.dead.add(s)
+        if s in self.order:
+            self.order.remove(s)
+        if not self._in_flight(s):
+            self._release(s)
+
+    def _release(self, s: int) -> None:
+        sl = self.slots.pop(s)
+        if s in self.order:
+            self.order.remove(s)
+        self.dead.discard(s)
+        self.cancelled.discard(s)
+        self.salloc.release(sl.soff, len(sl.req.t))
+        self.palloc.release(sl.poff, len(sl.req.periods))
+        self.dev.release(s)
+        self.free_slots.append(s)
+
+    def _admit(self, now: float):
+        new = []
+        while self.queue and self.free_slots and self.queue[0].arrival <= now:
+            req = self.queue[0]
+            so = self.salloc.alloc(len(req.t))
+            if so is None:
+                break
+            po = self.palloc.alloc(len(req.periods))
+            if po is None:
+                self.salloc.release(so, len(req.t))
+                break
+            self.queue.pop(0)
+            s = self.free_slots.pop()
+            self.slots[s] = _Slot(req, so, po, 0, now)
+            self.order.append(s)
+            try:
+                self.dev.upload(s, so, po, req)
+            except Exception as exc:  # this request alone fails
+                self._out.append(_failed(req, f"upload failed: {exc}", time.perf_counter()))
+                self._kill(s)
+                continue
+            new.append(s)
+        return new
+
+    # -- one step ------------------------------------------------------------
+    def _plan(self, new) -> StepPlan:
         cfg = self.cfg
-        now = time.perf_counter() if now is None else now
-        new = self._admit(now)
-        # uploads of newly admitted curves (pinned staging through torch)
-        for s in new:
-            sl = self.slots[s]
-            n, m = len(sl.req.t), len(sl.req.periods)
-            self.t[sl.soff:sl.soff + n].copy_(torch.from_numpy(
-                np.ascontiguousarray(sl.req.t, np.float32)), non_blocking=False)
-            self.x[sl.soff:sl.soff + n].copy_(torch.from_numpy(
-                np.ascontiguousarray(sl.req.flux, np.float32)), non_blocking=False)
-            self.per[sl.poff:sl.poff + m].copy_(torch.from_numpy(
-                np.ascontiguousarray(sl.req.periods, np.float64)), non_blocking=False)
-            self.h_off[s] = sl.soff
-            self.h_len[s] = n
-            self.h_poff[s] = sl.poff
-            self.h_np[s] = m
-        # detrend segments of the new curves
-        ns = 0
+        segs_c, segs_s = [], []
         for s in new:
             n = len(self.slots[s].req.t)
             k = (n + cfg.seg_len - 1) // cfg.seg_len
-            self.h_seg_curve[ns:ns + k] = s
-            self.h_seg_start[ns:ns + k] = torch.arange(0, k * cfg.seg_len, cfg.seg_len,
-                                                       dtype=torch.int32)
-            ns += k
-        # work items: oldest requests first, up to items_per_step
+            segs_c.append(np.full(k, s, np.int32))
+            segs_s.append(np.arange(0, k * cfg.seg_len, cfg.seg_len, dtype=np.int32))
         budget = cfg.items_per_step
         ic, ip, cost, done = [], [], [], []
         nep = 0
@@ -402,57 +660,79 @@ class TransitServer:
             budget -= take
             if sl.next_item == m:
                 done.append(s)
-        ni = cfg.items_per_step - budget
-        if ni:
+        if ic:
             ic, ip, cost = np.concatenate(ic), np.concatenate(ip), np.concatenate(cost)
             o = np.argsort(-cost, kind="stable")  # longest items first
-            self.h_item_curve[:ni] = torch.from_numpy(ic[o])
-            self.h_item_pidx[:ni] = torch.from_numpy(ip[o])
-        for k, s in enumerate(done):
-            self.h_stat_curves[k] = s
-        self.h_counts[0], self.h_counts[1], self.h_counts[2] = ns, ni, len(done)
-        # descriptors -> device (same stream, before the step's kernels)
-        for d, h in ((self.d_off, self.h_off), (self.d_len, self.h_len),
-                     (self.d_poff, self.h_poff), (self.d_np, self.h_np)):
-            d.copy_(h, non_blocking=True)
-        if ns:
-            self.d_seg_curve[:ns].copy_(self.h_seg_curve[:ns], non_blocking=True)
-            self.d_seg_start[:ns].copy_(self.h_seg_start[:ns], non_blocking=True)
-        if ni:
-            self.d_item_curve[:ni].copy_(self.h_item_curve[:ni], non_blocking=True)
-            self.d_item_pidx[:ni].copy_(self.h_item_pidx[:ni], non_blocking=True)
-        if done:
-            self.d_stat_curves[:len(done)].copy_(self.h_stat_curves[:len(done)],
-                                                 non_blocking=True)
-        self.d_counts[:3].copy_(self.h_counts[:3], non_blocking=True)
-        self._launch(nep * len(cfg.widths) * cfg.nl)
-        self.steps += 1
-        if not done:
-            torch.cuda.current_stream().synchronize()
-            return []
-        idx = torch.tensor(done, dtype=torch.int64, device="cuda")
-        best = self.d_best[idx].cpu().numpy()
-        bsn = self.d_bsn[idx].cpu().numpy()
-        sde = self.d_sde[idx].cpu().numpy()
-        bi = torch.from_numpy(np.maximum(best, 0)).cuda()
-        w = self.bw[bi].cpu().numpy()
-        l_ = self.bl[bi].cpu().numpy()
-        e = self.be[bi].cpu().numpy()
-        h = self.h[bi].cpu().numpy()
+            ic, ip = ic[o], ip[o]
+        else:
+            ic, ip = np.zeros(0, np.int32), np.zeros(0, np.int64)
+        slot_off = {s: (sl.soff, len(sl.req.t), sl.poff, len(sl.req.periods))
+                    for s, sl in self.slots.items()}
+        return StepPlan(new, np.concatenate(segs_c) if segs_c else np.zeros(0, np.int32),
+                        np.concatenate(segs_s) if segs_s else np.zeros(0, np.int32), ic, ip, done,
+                        slot_off, nep * len(cfg.widths) * cfg.nl)
+
+    def _collect_oldest(self) -> list[Result]:
+        """Wait for the oldest step in flight and retire its finished requests."""
+        handle, used, done = self.inflight.pop(0)
+        got = self.dev.collect(handle)
         tdone = time.perf_counter()
         out = []
-        for k, s in enumerate(done):
-            sl = self.slots.pop(s)
-            self.order.remove(s)
-            j = int(best[k]) - sl.poff if best[k] >= 0 else -1
-            out.append(Result(sl.req.rid, float(sl.req.periods[j]) if j >= 0 else float("nan"),
-                              j, float(bsn[k]), float(sde[k]),
-                              int(w[k]) if j >= 0 else -1, int(l_[k]) if j >= 0 else -1,
-                              int(e[k]) if j >= 0 else -1, float(h[k]) if j >= 0 else float("nan"),
-                              sl.req.arrival, tdone))
-            self.salloc.release(sl.soff, len(sl.req.t))
-            self.palloc.release(sl.poff, len(sl.req.periods))
-            self.free_slots.append(s)
+        for s in done:
+            sl = self.slots[s]
+            if s not in self.dead:
+                best, bsn, sde, w, l_, e, h, pg = got[s]
+                j = best - sl.poff if best >= 0 else -1
+                out.append(Result(sl.req.rid,
+                                  float(sl.req.periods[j]) if j >= 0 else float("nan"), j, bsn,
+                                  sde, w if j >= 0 else -1, l_ if j >= 0 else -1,
+                                  e if j >= 0 else -1, h if j >= 0 else float("nan"),
+                                  sl.req.arrival, tdone, periodogram=pg))
+            self._release(s)
+        for s in list(self.dead):
+            if s in self.slots and not self._in_flight(s):
+                self._release(s)
+        return out
+
+    def step(self, now: float | None = None) -> list[Result]:
+        """Admit arrived requests, plan and launch one device step, then
+        collect the step launched before it (pipelined: the device always has
+        the next step queued while the host collects and plans).  Returns the
+        results that became available; :meth:`flush` collects the rest."""
+        now = time.perf_counter() if now is None else now
+        new = self._admit(now)
+        plan = self._plan(new)
+        out = self._out
+        self._out = []
+        if not (len(plan.item_curve) or plan.done or plan.new):
+            out.extend(self.flush())
+            return out
+        used = set(plan.item_curve.tolist()) | set(plan.done) | set(new)
+        try:
+            handle = self.dev.launch(plan)
+        except Exception as exc:
+            # the step did not run: every request with work in it fails alone
+            tnow = time.perf_counter()
+            for s in sorted(used):
+                if s in self.slots and s not in self.dead:
+                    out.append(_failed(self.slots[s].req, f"step failed: {exc}", tnow))
+                    self._kill(s)
+            return out
+        self.steps += 1
+        self.inflight.append((handle, used, list(plan.done)))
+        if len(self.inflight) > 1:
+            out.extend(self._collect_oldest())
+        return out
+
+    def flush(self) -> list[Result]:
+        """Collect every step in flight."""
+        out = self._out
+        self._out = []
+        while self.inflight:
+            out.extend(self._collect_oldest())
+        for s in list(self.dead):
+            if s in self.slots:
+                self._release(s)
         return out
 
     def busy(self) -> bool:
@@ -469,8 +749,12 @@ class TransitServer:
         results = []
         while self.busy():
             now = time.perf_counter()
-            if self.queue and not self.slots and self.queue[0].arrival > now:
-                time.sleep(min(1e-3, self.queue[0].arrival - now))
+            pending = any(sl.next_item < len(sl.req.periods) for s, sl in self.slots.items()
+                          if s not in self.dead)
+            if self.queue and not pending and self.queue[0].arrival > now:
+                results.extend(self.flush())
+                time.sleep(min(1e-3, max(0.0, self.queue[0].arrival - time.perf_counter())))
                 continue
             results.extend(self.step(now))
+        results.extend(self.flush())
         return results

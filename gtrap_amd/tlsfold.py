@@ -2,13 +2,15 @@
 
 For each curve and each trial period P the light curve is folded, every epoch
 ``e < ceil(P/dtep)`` at ``t0 = e*dtep`` and every trapezoid (W, L) of the grid
-is scored with the statistic of gtrap/tls.py:173-208 (``get_chi2Hmax`` with
-``divx=True``), S/N = -H/sqrt(res/(n-3)) (tls.py:229), and the best trial of
+is scored with the statistic of gtrap/tls.py:44-79 (``get_chi2Hmax`` with
+``divx=True``), S/N = -H/sqrt(res/(n-3)) (tls.py:100), and the best trial of
 each period is kept (first maximum in (W, L, epoch) order, NaN skipped, as
-``tlsfit``'s nanargmax, tls.py:217-231).  The exact fold rule is
+``tlsfit``'s nanargmax, tls.py:82-110).  The exact fold rule is
 ``u = fmod(t, P)``, ``d = u - t0`` wrapped by +-P into [-P/2, P/2);
 ``L = m * 0.45 * W / nl`` (the reference's normal-mode L grid,
-include/gtls_simple.h:69,167).  Samples with negative time are masked.
+include/gtls_simple.h:69,167).  Samples with a negative time (masked) or a
+non-finite flux are skipped.  A curve longer than the batch's ``nmax`` is not
+searched (S/N NaN, indices -2); :func:`pack` sets ``nmax`` to the longest curve.
 
 The kernel is ``tlsfold_kernel`` (gtrap_amd/csrc/tlsfold.hip); this module
 packs a ragged batch of curves, owns the workspace and returns host results.
@@ -72,7 +74,16 @@ class Workspace:
         return self.buf, need
 
 
-_WS = Workspace()
+_WS: dict = {}
+
+
+def _workspace_for(stream) -> Workspace:
+    """Default workspace of a stream: the work ticket and the per-workgroup
+    slots live in it, so launches on different streams must not share one."""
+    key = _native.stream_handle(stream)
+    if key not in _WS:
+        _WS[key] = Workspace()
+    return _WS[key]
 
 
 def tlsfold_device(batch: FoldBatch, widths, nl: int, dtep: float, out=None,
@@ -81,7 +92,7 @@ def tlsfold_device(batch: FoldBatch, widths, nl: int, dtep: float, out=None,
     """Launch on a packed batch; returns device tensors (sn, h, w, l, e)."""
     import torch
     _native.require_gpu()
-    ws = workspace or _WS
+    ws = workspace or _workspace_for(stream)
     nb = max(1, min(int(nblocks), max(1, batch.nptot)))
     buf, need = ws.get(batch.nmax, nb)
     wdev = widths if isinstance(widths, torch.Tensor) else \

@@ -281,20 +281,24 @@ int gtrap_tlsgrid(const double *t, const double *x, int32_t n, const double *Wl,
 /* ---- periodic trapezoid least squares (tls_fold) --------------------------- */
 
 /*
- * Periodic extension of the trapezoid statistic of gtrap/tls.py:173-231
+ * Periodic extension of the trapezoid statistic of gtrap/tls.py:44-110
  * (get_chi2Hmax with divx=True, tlsfit's S/N = -H/sqrt(res/(n-3)) and
  * nanargmax over the grid).  For every trial period P (float64, days) of a
  * curve: u = fmod(t, P), epochs e < ceil(P/dtep) at t0 = e*dtep, offset
  * d = u - t0 wrapped by +-P to [-P/2, P/2); widths W = widths[w] and
  * L = m*(0.45*W/nl), m < nl (the L grid of include/gtls_simple.h:69,167).
- * Samples with t < 0 are masked.  A trial with n <= 3 samples in its window
- * or with W >= P is NaN.  Best trial per period = first maximum in
- * (w, m, e) order, NaN skipped.
+ * Samples with t < 0 (masked) or a non-finite flux are skipped.  A trial
+ * with n <= 3 samples in its window or with W >= P is NaN.  Best trial per
+ * period = first maximum in (w, m, e) order, NaN skipped.
  *
  * Ragged batch: curve c has len[c] samples at offset off[c] of the flat t, x
- * arrays (nmax >= every len[c]); its periods are periods[poff[c] .. poff[c+1])
- * with poff[ncurves] = nptot.  Outputs per period (index poff[c] + p): best
- * S/N, its H, and the w, m, e indices (-1 when every trial is NaN).
+ * arrays; its periods are periods[poff[c] .. poff[c+1]) with poff[ncurves] =
+ * nptot.  Outputs per period (index poff[c] + p): best S/N, its H, and the
+ * w, m, e indices (-1 when every trial is NaN).
+ * Precondition len[c] <= nmax: each workgroup's workspace slot holds nmax
+ * samples.  The lengths live on the device, so the kernel checks it: the
+ * periods of a longer curve are not searched and get S/N NaN, H NaN and
+ * indices -2.
  * ws: caller workspace of at least gtrap_tlsfold_ws_bytes(nmax, nblocks)
  * bytes; nblocks = persistent workgroups (execution parameter).
  */
@@ -312,7 +316,8 @@ int gtrap_tlsfold(double *snr, double *hbest, int32_t *best_w, int32_t *best_l,
  * and the outputs) of curve slot item_curve[k] (off/len tables); the item
  * count is *nitems_dev and items are handed out through the atomic ticket
  * *counter, which the caller zeroes before the launch.  nblocks persistent
- * workgroups; launch shape fixed (HIP-graph capturable).
+ * workgroups; launch shape fixed (HIP-graph capturable).  Same sample rules
+ * and len[c] <= nmax precondition as gtrap_tlsfold.
  */
 int gtrap_tlsfold_items(double *snr, double *hbest, int32_t *best_w, int32_t *best_l,
                         int32_t *best_e, const float *t, const float *x, const int64_t *off,
@@ -326,8 +331,10 @@ int gtrap_tlsfold_items(double *snr, double *hbest, int32_t *best_w, int32_t *be
  * Periodogram statistics of the tls_fold output, the TLS counterpart of
  * gtrap/getstat.py:100-123: for curve slots curves[0 .. *ncurves_dev) (at most
  * max_curves), over periods [poff[c], poff[c] + np[c]): best_pidx[c] = index
- * of the first maximum S/N (NaN/-inf skipped, -1 if none), best_sn[c], and
- * sde[c] = (best - mean)/std over the finite S/N values.  float64.
+ * of the first maximum S/N over the values that are neither NaN nor -inf
+ * (-inf: a period without any trial; +inf, a perfect fit, can win; -1 if
+ * none), best_sn[c], and sde[c] = (best - mean)/std with the mean and
+ * standard deviation over the finite S/N values.  float64.
  */
 int gtrap_tlsstats(const double *sn, const int64_t *poff, const int32_t *np_,
                    const int32_t *curves, const int32_t *ncurves_dev, int32_t max_curves,

@@ -164,15 +164,19 @@ void orc_geebls(float *blsp, uint32_t *phase1, uint32_t *phase2, const float *lc
 /* Folding: u = fmod(t, P) in float64 (exact); epoch e has t0 = e*dtep,      */
 /* e < ceil(P/dtep).  Offset of a sample: d = u - t0, replaced by            */
 /* (u - P) - t0 when d >= P/2 and by (u + P) - t0 when d < -P/2 (each        */
-/* operation rounded in float64).  Masked samples (t < 0) are skipped.      */
-/* Trial (W, L = m * (0.45*W/nl)): regions of tls.py:175-179 on d,           */
+/* operation rounded in float64).  Masked samples (t < 0) and samples with   */
+/* a non-finite flux are skipped.                                           */
+/* Trial (W, L = m * (0.45*W/nl)): regions of tls.py:46-50 on d,             */
 /* A [-W/2, -W/4-L/2), B [-W/4-L/2, -W/4+L/2), C [-W/4+L/2, W/4-L/2),        */
-/* D [W/4-L/2, W/4+L/2), E [W/4+L/2, W/2); mean over [-W/2, W/2] (:187);     */
-/* denA, numB, H = -numB/denA (:200-202); res (:204-206);                   */
+/* D [W/4-L/2, W/4+L/2), E [W/4+L/2, W/2); mean over [-W/2, W/2] (:57-59);   */
+/* denA, numB, H = -numB/denA (:71-73); res (:74-77);                       */
 /* S/N = -H / sqrt(res / (n - 3)) (tls.py:100); a trial with n <= 3 (no      */
 /* degrees of freedom) is NaN, and so is one with W >= P (a window as long  */
 /* as the period would hold two images of a sample).                        */
 /* ------------------------------------------------------------------------ */
+/* a sample takes part when its time is valid (t >= 0) and its flux finite */
+static int usable(float t, float x) { return t >= 0.0f && isfinite(x); }
+
 static double fold_offset(double u, double t0, double P) {
   double d = u - t0;
   if (d >= 0.5 * P) d = (u - P) - t0;
@@ -181,7 +185,8 @@ static double fold_offset(double u, double t0, double P) {
 }
 
 double orc_tlsfold_trial(const float *t, const float *x, int n, double P, int e, double W,
-                         int m, int nl, double dtep, int divx, double *Hout, int *cnt) {
+                         int m, int nl, double dtep, int divx, double *Hout, int *cnt,
+                         double *resout, double *meanout) {
   const double t0 = (double)e * dtep;
   const double dl = 0.45 * W / (double)nl;
   const double L = (double)m * dl;
@@ -190,15 +195,15 @@ double orc_tlsfold_trial(const float *t, const float *x, int n, double P, int e,
   double msum = 0.0;
   int mcnt = 0;
   for (int i = 0; i < n; ++i) {
-    if (!(t[i] >= 0.0f)) continue;
+    if (!usable(t[i], x[i])) continue;
     const double d = fold_offset(fmod((double)t[i], P), t0, P);
     if (d >= -hw && d <= hw) { msum += (double)x[i]; mcnt++; }
   }
-  const double mean = divx ? msum / (double)mcnt : 0.0; /* divx=False: tls.py:189-191 */
+  const double mean = divx ? msum / (double)mcnt : 0.0; /* divx=False: tls.py:60-62 */
   int nA = 0, nB = 0, nC = 0, nD = 0, nE = 0;
   double xAEC = 0, xBD = 0, xtB = 0, xtD = 0, tB = 0, tD = 0, t2BD = 0;
   for (int i = 0; i < n; ++i) {
-    if (!(t[i] >= 0.0f)) continue;
+    if (!usable(t[i], x[i])) continue;
     const double d = fold_offset(fmod((double)t[i], P), t0, P);
     const double y = (double)x[i] - mean;
     if (d >= -hw && d < tAB) { nA++; xAEC += y; }
@@ -214,7 +219,7 @@ double orc_tlsfold_trial(const float *t, const float *x, int n, double P, int e,
   const double h = H / 2.0;
   double res = 0.0;
   for (int i = 0; i < n; ++i) {
-    if (!(t[i] >= 0.0f)) continue;
+    if (!usable(t[i], x[i])) continue;
     const double d = fold_offset(fmod((double)t[i], P), t0, P);
     const double y = (double)x[i] - mean;
     double r;
@@ -229,6 +234,8 @@ double orc_tlsfold_trial(const float *t, const float *x, int n, double P, int e,
   const int nn = nA + nB + nC + nD + nE;
   if (Hout) *Hout = H;
   if (cnt) *cnt = nn;
+  if (resout) *resout = res;
+  if (meanout) *meanout = mean;
   if (nn <= 3) return NAN; /* n - 3 degrees of freedom: no trial below 4 samples */
   if (!(W < P)) return NAN; /* no trial: the window spans the period */
   return -H / sqrt(res / (double)(nn - 3));
@@ -308,7 +315,7 @@ void orc_tlsfold(const float *t, const float *x, int n, const double *periods, i
     const double P = periods[p];
     int k = 0;
     for (int i = 0; i < n; ++i) {
-      if (!(t[i] >= 0.0f)) continue;
+      if (!usable(t[i], x[i])) continue;
       s[k].u = fmod((double)t[i], P);
       s[k].x = (double)x[i];
       k++;

@@ -1,21 +1,23 @@
 """Periodic trapezoid least squares oracle (fold extension of gtrap/tls.py).
 
 Definition (shared with gtrap_amd/tlsfold.py and the HIP kernel):
-for each trial period P, every valid sample (t >= 0) is folded to
+for each trial period P, every usable sample (t >= 0, finite flux) is folded to
 u = fmod(t, P) (float64, exact); epoch e < ceil(P/dtep) sits at t0 = e*dtep;
 the sample's offset is d = u - t0, replaced by (u - P) - t0 when d >= P/2 and
 by (u + P) - t0 when d < -P/2.  For each width W and L = m*(0.45*W/nl)
 (m < nl; the reference's normal-mode grid, include/gtls_simple.h:69,167) the
-statistic of gtrap/tls.py:173-208 (get_chi2Hmax with divx=True, sigma=1) is
-evaluated on d, and S/N = -H/sqrt(res/(n-3)) (tls.py:229); a trial with
+statistic of gtrap/tls.py:44-79 (get_chi2Hmax with divx=True, sigma=1) is
+evaluated on d, and S/N = -H/sqrt(res/(n-3)) (tls.py:100); a trial with
 n <= 3 samples in [-W/2, W/2) has no degrees of freedom and is NaN, and so is
 a trial with W >= P (its window would hold two images of a sample).  The best trial of
 a period is the first maximum in (W, L, epoch) order, NaN skipped
-(tls.py:217-231, nanargmax).
+(tls.py:82-110, nanargmax).
 
 * :func:`tlsfold` -- C, direct per-trial sums over the window samples.
 * :func:`trial` -- C, one trial from the raw arrays (no candidate selection).
+* :func:`trial_res` -- as :func:`trial`, also the residual and window mean.
 * :func:`trial_numpy` -- the tls.py masks evaluated with numpy on folded d.
+* :func:`tlsstats` -- periodogram statistics of one curve (best period, SDE).
 """
 from __future__ import annotations
 
@@ -44,23 +46,40 @@ def tlsfold(t, x, periods, widths, nl, dtep):
     return dict(sn=sn, w=bw, l=bl, e=be, h=h)
 
 
-def trial(t, x, P, e, W, m, nl, dtep, divx=True):
+def _trial(t, x, P, e, W, m, nl, dtep, divx):
     t = np.ascontiguousarray(t, np.float32)
     x = np.ascontiguousarray(x, np.float32)
     f = lib().orc_tlsfold_trial
     f.restype = C.c_double
     f.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int,
-                  C.c_int, C.c_double, C.c_int, C.c_void_p, C.c_void_p]
+                  C.c_int, C.c_double, C.c_int] + [C.c_void_p] * 4
     H = np.zeros(1)
     cnt = np.zeros(1, np.int32)
+    res = np.zeros(1)
+    mean = np.zeros(1)
     sn = f(fptr(t), fptr(x), len(t), float(P), int(e), float(W), int(m), int(nl), float(dtep),
-           1 if divx else 0, fptr(H), fptr(cnt))
-    return sn, H[0], int(cnt[0])
+           1 if divx else 0, fptr(H), fptr(cnt), fptr(res), fptr(mean))
+    return sn, H[0], int(cnt[0]), res[0], mean[0]
 
 
-def fold_offsets(t, P, e, dtep):
+def trial(t, x, P, e, W, m, nl, dtep, divx=True):
+    """One trial: (S/N, H, n)."""
+    return _trial(t, x, P, e, W, m, nl, dtep, divx)[:3]
+
+
+def trial_res(t, x, P, e, W, m, nl, dtep, divx=True):
+    """One trial: (S/N, H, n, res, mean) with res the residual sum of squares
+    of tls.py:74-77 and mean the window mean of tls.py:57-59 (0 without divx)."""
+    return _trial(t, x, P, e, W, m, nl, dtep, divx)
+
+
+def fold_offsets(t, P, e, dtep, x=None):
+    """Offsets d of the usable samples (t >= 0, and finite flux when x is
+    given) from epoch e of period P; returns (d, mask of the usable samples)."""
     t = np.asarray(t)
     ok = t >= 0
+    if x is not None:
+        ok &= np.isfinite(np.asarray(x))
     u = np.fmod(t[ok].astype(np.float64), P)
     t0 = float(e) * dtep
     d = u - t0
@@ -69,7 +88,7 @@ def fold_offsets(t, P, e, dtep):
 
 
 def chi2hmax_numpy(tn, xx, W, L, divx=True, sigma=1.0):
-    """get_chi2Hmax of tls.py:173-208 on offsets tn (float64 numpy, nan-aware).
+    """get_chi2Hmax of tls.py:44-79 on offsets tn (float64 numpy, nan-aware).
     Returns (Hmax, res, n, xsum)."""
     tn = np.asarray(tn, np.float64)
     xx = np.asarray(xx, np.float64)
@@ -103,12 +122,37 @@ def chi2hmax_numpy(tn, xx, W, L, divx=True, sigma=1.0):
 
 
 def trial_numpy(t, x, P, e, W, m, nl, dtep, divx=True):
-    """tls.py's get_chi2Hmax masks and formulas (tls.py:173-208) on folded d;
-    returns (S/N, H, n) with S/N = -H/sqrt(res/(n-3)) (tls.py:229)."""
-    d, ok = fold_offsets(t, P, e, dtep)
+    """tls.py's get_chi2Hmax masks and formulas (tls.py:44-79) on folded d;
+    returns (S/N, H, n) with S/N = -H/sqrt(res/(n-3)) (tls.py:100)."""
+    d, ok = fold_offsets(t, P, e, dtep, x)
     xx = np.asarray(x)[ok].astype(np.float64)
     L = m * (0.45 * W / nl)
     H, res, n, _ = chi2hmax_numpy(d, xx, W, L, divx=divx)
     with np.errstate(all="ignore"):
         sn = -H / np.sqrt(res / (n - 3)) if n > 3 else np.nan
     return sn, H, n
+
+
+def tlsstats(sn):
+    """Periodogram statistics of one curve's per-period S/N (float64).
+
+    best: the first maximum over the values that are neither NaN nor -inf
+    (-inf marks a period without any trial; +inf, a perfect fit, can win),
+    -1 when there is none; mean and variance over the finite values;
+    SDE = (best value - mean) / sqrt(var), the statistic of
+    gtrap/getstat.py:114-118 (ave = sum/n, var = sumsq/n - ave^2) applied to
+    the trapezoid periodogram.  Returns (best index, best value, SDE)."""
+    sn = np.asarray(sn, np.float64)
+    cand = ~np.isnan(sn) & ~np.isneginf(sn)
+    if not cand.any():
+        best, bval = -1, -np.inf
+    else:
+        v = np.where(cand, sn, -np.inf)
+        best = int(np.argmax(v))
+        bval = float(v[best])
+    fin = sn[np.isfinite(sn)]
+    with np.errstate(all="ignore"):
+        mean = fin.sum() / len(fin) if len(fin) else np.nan
+        var = (fin * fin).sum() / len(fin) - mean * mean if len(fin) else np.nan
+        sde = (bval - mean) / np.sqrt(var)
+    return best, bval, float(sde)

@@ -126,3 +126,129 @@ def test_gapped_curve(gpu):
     x = 1.0 + 1e-3 * rng.standard_normal(t.size)
     _check([(t.astype(np.float32), x.astype(np.float32))],
            [np.array([0.9, 1.3, 2.7, 6.6, 11.0, 17.5, 40.0])], np.linspace(0.04, 0.3, 10), 10)
+
+
+def test_high_snr_expanded_residual(gpu):
+    """A trapezoid of the trial grid itself (depth 1e-2) at noise 1e-6, sampled
+    at the midpoints of a grid aligned to its region boundaries at every cycle
+    (cadence 1/64 d, P = 256 cadences): the model matches up to the noise, so
+    the best trial's residual is ~4e-8 of the window's sum of squares and the
+    kernel's expanded residual (y2 denA^2 - numB denA U + numB^2 V) cancels by
+    seven digits.  S/N (~1e4) must still match the direct oracle within 1e-5,
+    with the same best trials."""
+    from gtrap_amd import tlsfold
+    rng = np.random.default_rng(41)
+    dt = 1.0 / 64
+    P, W, nl, m, e = 4.0, 2.0, 9, 5, 100
+    L = m * (0.45 * W / nl)
+    t = (np.arange(25 * 256) + 0.5) * dt
+    tn = (t - e * dt + P / 2) % P - P / 2
+    g = np.zeros_like(tn)
+    g[tn <= -W / 4 - L / 2] = -0.5
+    g[tn >= W / 4 + L / 2] = -0.5
+    g[(tn < W / 4 - L / 2) & (tn >= -W / 4 + L / 2)] = 0.5
+    mB = (tn > -W / 4 - L / 2) & (tn <= -W / 4 + L / 2)
+    g[mB] = (tn[mB] + W / 4) / L
+    mD = (tn < W / 4 + L / 2) & (tn >= W / 4 - L / 2)
+    g[mD] = -(tn[mD] - W / 4) / L
+    x = 1.0 - 1e-2 * g + 1e-6 * rng.standard_normal(t.size)
+    t, x = t.astype(np.float32), x.astype(np.float32)
+    per = np.array([3.9, 4.0, 4.1])
+    widths = np.array([1.0, 1.5, 2.0])
+    got = tlsfold.tlsfold([(t, x)], [per], widths, nl, dt, nblocks=16)[0]
+    want = ot.tlsfold(t, x, per, widths, nl, dt)
+    assert (want["w"][1], want["l"][1], want["e"][1]) == (2, m, e)
+    assert want["sn"][1] > 1e3
+    np.testing.assert_allclose(got["sn"], want["sn"], rtol=1e-5)
+    np.testing.assert_array_equal(got["w"], want["w"])
+    np.testing.assert_array_equal(got["l"], want["l"])
+    np.testing.assert_array_equal(got["e"], want["e"])
+    np.testing.assert_allclose(got["h"], want["h"], rtol=1e-6)
+
+
+def test_noiseless_aligned_trapezoid_known_answer(gpu):
+    """An exact trapezoid (tls.py:2-15) sampled at the midpoints of a grid
+    aligned to its region boundaries has zero mean over its window, so the
+    mean-removed fit recovers H exactly and the residual is 0: the search must
+    pick exactly that trial (S/N infinite or huge) with H = -2^-6, for a
+    period above the span of the data (the fold is the identity)."""
+    from gtrap_amd import tlsfold
+    dt = 1.0 / 64
+    W, nl, m, e = 2.0, 9, 5, 400
+    L = m * (0.45 * W / nl)
+    t0 = e * dt
+    k = np.arange(-64, 64)
+    t = t0 + (k + 0.5) * dt
+    H = -2.0 ** -6
+    tn = t - t0
+    g = np.zeros_like(tn)
+    g[tn <= -W / 4 - L / 2] = -0.5
+    g[tn >= W / 4 + L / 2] = -0.5
+    g[(tn < W / 4 - L / 2) & (tn >= -W / 4 + L / 2)] = 0.5
+    mB = (tn > -W / 4 - L / 2) & (tn <= -W / 4 + L / 2)
+    g[mB] = (tn[mB] + W / 4) / L
+    mD = (tn < W / 4 + L / 2) & (tn >= W / 4 - L / 2)
+    g[mD] = -(tn[mD] - W / 4) / L
+    x = H * g
+    t32, x32 = t.astype(np.float32), x.astype(np.float32)
+    assert np.array_equal(t32, t) and np.array_equal(x32, x)
+    widths = np.array([1.0, 2.0])
+    per = np.array([16.0])
+    got = tlsfold.tlsfold([(t32, x32)], [per], widths, nl, dt, nblocks=8)[0]
+    want = ot.tlsfold(t32, x32, per, widths, nl, dt)
+    for r in (got, want):
+        assert (r["w"][0], r["l"][0], r["e"][0]) == (1, m, e)
+        assert abs(r["h"][0] - H) <= 1e-12 * abs(H)
+        assert r["sn"][0] == np.inf or r["sn"][0] > 1e6
+
+
+def test_nonfinite_flux_skipped(gpu):
+    """NaN and +-inf flux values are skipped like masked samples, as in the
+    oracle (the windows around them keep their other samples)."""
+    t, x = _curve(3000, 51, mask=0.02)
+    rng = np.random.default_rng(52)
+    bad = rng.choice(3000, 40, replace=False)
+    x = x.copy()
+    x[bad[:20]] = np.nan
+    x[bad[20:30]] = np.inf
+    x[bad[30:]] = -np.inf
+    x[0] = np.nan  # the first sample (the flux reference) is skipped too
+    _check([(t, x)], [np.geomspace(0.5, 8.0, 17)], np.linspace(0.04, 0.3, 10), 10)
+    tm = t.copy()
+    tm[~np.isfinite(x)] = -1000.0
+    from gtrap_amd import tlsfold
+    a = tlsfold.tlsfold([(t, x)], [np.geomspace(0.5, 8.0, 17)], np.linspace(0.04, 0.3, 10))[0]
+    b = tlsfold.tlsfold([(tm, x)], [np.geomspace(0.5, 8.0, 17)], np.linspace(0.04, 0.3, 10))[0]
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k])
+
+
+def test_clustered_phases_larger_than_a_sort_chunk(gpu):
+    """20000 samples sharing 2 exact time stamps (so 2 phases at every
+    period) plus a regular curve: one phase bucket holds 10000 samples, more
+    than an LDS sort chunk, and is ordered by the workgroup's bitonic sort."""
+    rng = np.random.default_rng(61)
+    tr = np.arange(3000) * 0.0204
+    t = np.concatenate([np.full(10000, 12.3), tr, np.full(10000, 40.1)])
+    x = 1.0 + 1e-3 * rng.standard_normal(t.size)
+    _check([(t.astype(np.float32), x.astype(np.float32))], [np.array([0.7, 1.3, 2.9, 9.7])],
+           np.linspace(0.04, 0.3, 10), 10)
+
+
+def test_curve_longer_than_the_workspace_slot_is_refused(gpu):
+    """A curve longer than nmax is not searched (S/N NaN, indices -2); the
+    others of the batch are."""
+    import torch
+    from gtrap_amd import _native, tlsfold
+    curves = [_curve(500, 71), _curve(900, 72)]
+    per = [np.array([0.7, 1.9]), np.array([0.8, 2.2, 3.3])]
+    b = tlsfold.pack(curves, per)
+    b.nmax = 600  # the second curve does not fit its slot
+    out = tlsfold.tlsfold_device(b, np.linspace(0.04, 0.3, 10), 10, 0.0204, nblocks=4,
+                                 workspace=tlsfold.Workspace())
+    torch.cuda.synchronize()
+    sn = out["sn"].cpu().numpy()
+    w = out["w"].cpu().numpy()
+    assert np.all(np.isnan(sn[2:])) and np.all(w[2:] == -2)
+    want = ot.tlsfold(*curves[0], per[0], np.linspace(0.04, 0.3, 10), 10, 0.0204)
+    np.testing.assert_allclose(sn[:2], want["sn"], rtol=1e-7)

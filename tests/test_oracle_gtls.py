@@ -62,3 +62,55 @@ def test_absolute_mode_uses_abs_h():
     ab = og.gtls(x[:, None], t[:, None], mode="absolute", **kw)
     assert ab["tlssn"][80, 0] > 0 and ab["tlshmax"][80, 0] < 0
     assert ab["tlssn"][80, 0] > norm["tlssn"][80, 0]
+
+
+def _gtls_scoop(lc, tu, k, nsc, deltat):
+    """Scoop of epoch k as include/gtls_simple.h:51-65,121-134 builds it:
+    rows k - nsc/2 .. k - nsc/2 + nsc - 1, stored time -1000 + fl(tu + 1000)
+    (float32), masked rows (stored time < 0 or outside the curve) dropped,
+    shifted time tau = fl(T - deltat*k) with one rounding (a fused multiply-add)."""
+    nt = len(lc)
+    rows = np.arange(k - nsc // 2, k - nsc // 2 + nsc)
+    rows = rows[(rows >= 0) & (rows < nt)]
+    T = (np.float32(-1000.0) + (tu[rows].astype(np.float32) + np.float32(1000.0))).astype(
+        np.float32)
+    ok = T >= 0
+    tau = (T[ok].astype(np.float64) - np.float64(np.float32(deltat)) * k).astype(np.float32)
+    return tau.astype(np.float64), lc[rows][ok].astype(np.float64)
+
+
+def test_residual_identity_of_the_best_trial():
+    """The reported S/N = H/sqrt(res/(n-3)) (gtls_simple.h:359-369) implies a
+    residual; it equals the sum over the window (-W/2, W/2] of
+    (x - mean - gen_trapzoid(tau, H, W, L, 0))^2 with the reported H, W, L
+    (mean over the closed window, gtls_simple.h:139-152).  The model of
+    tls.py:2-15 is continuous, so region membership at the boundaries does not
+    enter; float32 arithmetic bounds the agreement."""
+    from tests.test_oracle_tls_pins import gen_trapzoid
+    rng = np.random.default_rng(21)
+    nt = 600
+    t = (np.arange(nt) * 0.0204).astype(np.float32)
+    x = (1e-3 * rng.standard_normal(nt)).astype(np.float32)
+    x[(t > 5.0) & (t < 5.12)] += 3e-3
+    tu = t.copy()
+    tu[300:310] = -1000.0
+    nw, nl, wmax, dw, deltat = 8, 6, 0.3, 0.03, np.float32(0.0204)
+    nsc = int(wmax / deltat + 3)
+    out = og.gtls(x[:, None], tu[:, None], nw, nl, nsc, wmax, dw, deltat)
+    checked = 0
+    for k in range(20, nt - 20, 7):
+        sn = float(out["tlssn"][k, 0])
+        if sn <= 0:
+            continue
+        W, L, H = (float(out[n][k, 0]) for n in ("tlsw", "tlsl", "tlshmax"))
+        tau, xx = _gtls_scoop(x, tu, k, nsc, deltat)
+        W32 = np.float32(W)
+        lo, hi = np.float32(-0.5 * W32), np.float32(0.5 * W32)
+        mean = xx[(tau >= lo) & (tau <= hi)].mean()
+        win = (tau > lo) & (tau <= hi)
+        r = xx[win] - mean - gen_trapzoid(tau[win], H, W, L, 0.0)
+        res_direct = float(np.sum(r * r))
+        res_implied = (win.sum() - 3) * (H / sn) ** 2
+        np.testing.assert_allclose(res_implied, res_direct, rtol=2e-4)
+        checked += 1
+    assert checked > 30
