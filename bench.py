@@ -32,6 +32,9 @@ TRIAL_FLOPS = 103
 # tlsfold_kernel<true>, scaled by the same counters over a 1 GiB copy
 # (factors 2.0 and 1.0)
 SERVE_FOLD_TRAFFIC = 1.317e11
+SERVE_FOLD_TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this workload "
+                             "before the current build (not measured in this run), "
+                             "profiles/tlsfold_serve_hbm_traffic.md")
 
 
 def log(msg: str) -> None:
@@ -185,11 +188,17 @@ def bench_geebls(steps: int, warmup: int):
     )
 
 
+def cpu_trial_rates(t, x, periods, widths, nl, dtep, cap=8.0):
+    """CPU baselines of the trapezoid search (the C oracle and the numpy
+    restatement of tls.py, one core and a process pool)."""
+    from oracle import timing
+    return timing.trial_rates(t, x, periods, widths, nl, dtep, cap=cap, log=log)
+
+
 # ----------------------------------------------------------------------------
-# serving workload: TLS + detrend requests, served vs plain
+# serving workload: TLS+detrend requests, served vs plain
 # ----------------------------------------------------------------------------
 SERVE_REQUESTS = int(os.environ.get("GTRAP_BENCH_REQUESTS", "160"))
-PLAIN_REQUESTS = int(os.environ.get("GTRAP_BENCH_PLAIN", "24"))
 
 
 def bench_serve(steps: int, warmup: int):
@@ -199,19 +208,20 @@ def bench_serve(steps: int, warmup: int):
     cfg = ServerConfig.tuned()
     log(f"serve: building traces of {SERVE_REQUESTS} requests; items/step {cfg.items_per_step}, "
         f"workgroups {cfg.nblocks}")
-    traces = [synth.serving_trace(SERVE_REQUESTS, seed=1004 + k)[0]
+    traces = [synth.serving_trace(SERVE_REQUESTS, seed=1004 + k)
               for k in range(warmup + steps)]
     srv = TransitServer(cfg)
     srv.fold_log = []
     # warm-up (graph capture, allocator, code objects)
     for k in range(warmup):
-        srv.run(traces[k], realtime=True)
+        srv.run(traces[k][0], realtime=True)
     torch.cuda.synchronize()
     # served, Poisson arrivals in real time
     walls, lat, trials, nreq = [], [], 0, 0
-    kevs = []
+    rec, ninj = 0, 0
+    last = None
     for k in range(warmup, warmup + steps):
-        reqs = traces[k]
+        reqs, truth = traces[k]
         trials += sum(synth.trial_count(r) for r in reqs)
         nreq += len(reqs)
         t0 = time.perf_counter()
@@ -219,25 +229,36 @@ def bench_serve(steps: int, warmup: int):
         torch.cuda.synchronize()
         walls.append(time.perf_counter() - t0)
         lat += [r.latency for r in res]
+        for r in res:
+            if truth[r.rid] is not None:
+                ninj += 1
+                rec += synth.recovered(r.best_period, truth[r.rid])
+        last = {r.rid: r for r in res}
         log(f"served trace {k}: {len(res)} requests in {walls[-1]:.3f} s, {srv.steps} steps")
     wall = sum(walls)
     served = nreq / wall
     lat = np.array(lat)
-    # plain: one launch sequence per curve on a subset of the last trace
+    # plain: one launch sequence per curve over the same requests as the last
+    # served trace (every request of it)
     ctx = PlainSearch()
-    sub = traces[-1][:PLAIN_REQUESTS]
+    sub = traces[-1][0]
     search_plain(sub[0], cfg.widths, cfg.nl, cfg.dtep, cfg.r, ctx=ctx)  # warm
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     plat = []
+    same = True
     for r in sub:
         a = time.perf_counter()
-        search_plain(r, cfg.widths, cfg.nl, cfg.dtep, cfg.r, ctx=ctx)
+        p = search_plain(r, cfg.widths, cfg.nl, cfg.dtep, cfg.r, ctx=ctx)
         plat.append(time.perf_counter() - a)
+        g = last[r.rid]
+        same &= (p.best_index, p.w, p.l, p.e, p.sn) == (g.best_index, g.w, g.l, g.e, g.sn)
     pwall = time.perf_counter() - t0
     plain = len(sub) / pwall
     ptrials = sum(synth.trial_count(r) for r in sub)
-    log(f"plain: {len(sub)} requests in {pwall:.3f} s")
+    served_last = len(sub) / walls[-1]
+    log(f"plain: {len(sub)} requests in {pwall:.3f} s (served the same in {walls[-1]:.3f} s); "
+        f"results identical: {same}")
     # dominant kernel: the serving steps' tlsfold (item-list instantiation),
     # every launch of the run (warm-up included, as a profiler sees them)
     # bracketed by HIP events on the serving stream
@@ -245,55 +266,168 @@ def bench_serve(steps: int, warmup: int):
     # trials with L = 0 are NaN by construction and are not evaluated
     evaluated = ktr * (cfg.nl - 1) / cfg.nl
     ach = TRIAL_FLOPS * evaluated / ksec / 1e12
-    # CPU baseline: the direct oracle on a bounded sample of periods
-    from oracle import tlsfold as ot
+    # CPU baselines on one mid-size request of the last trace (scipy detrend +
+    # the trapezoid search), scaled by trial evaluations to the mean request
     from oracle import medfilt as om
     cores, model = cpu_info()
-    r0 = sub[1] if len(sub) > 1 else sub[0]
+    r0 = sorted(sub, key=lambda r: len(r.t))[len(sub) // 2]
     t1 = time.perf_counter()
     xd = om.detrend_scipy(r0.flux[:, None], cfg.r, "div")[:, 0]
-    done = 0
-    cpu_trials = 0
-    idx = np.linspace(0, len(r0.periods) - 1, 64).astype(int)
-    for j in idx:
-        ot.tlsfold(r0.t, xd, r0.periods[j:j + 1], cfg.widths, cfg.nl, cfg.dtep)
-        cpu_trials += int(np.ceil(r0.periods[j] / cfg.dtep)) * len(cfg.widths) * cfg.nl
-        done += 1
-        if time.perf_counter() - t1 > 20.0:
-            break
-    cpu_s = time.perf_counter() - t1
-    cpu_tps = cpu_trials / cpu_s
-    cpu_lcs = cpu_tps / (trials / nreq)
+    det_s = time.perf_counter() - t1
+    rates = cpu_trial_rates(r0.t, xd, r0.periods, cfg.widths, cfg.nl, cfg.dtep)
+    # light curves/s of the request's size class, scaled by trial evaluations
+    # to the trace's mean request
+    scale = synth.trial_count(r0) / (trials / nreq)
+    lcs = lambda curves: 1.0 / (scale / curves + det_s)
     return dict(
         metric="light curves/s and trial evals/s (TLS+detrend), served vs plain; p99 latency",
         value=served, unit="light curves/s", ms_per_step=wall / steps * 1e3, dtype="f64",
         trial_evals_per_s=trials / wall,
         plain=dict(value=plain, unit="light curves/s", trial_evals_per_s=ptrials / pwall,
-                   requests=len(sub), p50_ms=float(np.median(plat) * 1e3),
+                   requests=len(sub), same_requests_as="the last served trace",
+                   results_identical=bool(same), p50_ms=float(np.median(plat) * 1e3),
                    p99_ms=float(np.percentile(plat, 99) * 1e3)),
-        served_vs_plain=served / plain,
+        served_vs_plain=dict(light_curves=served_last / plain,
+                             trial_evals=(ptrials / walls[-1]) / (ptrials / pwall),
+                             target=10.0, met=bool(served_last / plain >= 10.0)),
         latency_ms=dict(p50=float(np.percentile(lat, 50) * 1e3),
                         p99=float(np.percentile(lat, 99) * 1e3),
-                        max=float(lat.max() * 1e3)),
+                        max=float(lat.max() * 1e3),
+                        note="arrival to result of requests arriving in 160-request bursts "
+                             "(80 ms at 2000 req/s) that drain before the next burst; "
+                             "open-loop figures: --workload openloop"),
+        detection=dict(injected=ninj, recovered=rec, fraction=rec / max(ninj, 1),
+                       rule="best period within 1% of P, P/2, P/3, 2P or 3P"),
         config=dict(workload="serving ragged TLS+detrend: Poisson 2000 req/s, n log-uniform "
                              "2048-70000 at 0.0204 d, periods geomspace(0.5, baseline/3, "
                              "500-4000), 10 W x 10 L, detrend r=64 divide",
                     requests_per_step=SERVE_REQUESTS, items_per_step=cfg.items_per_step,
                     nblocks=cfg.nblocks, hbm_budget_gib=cfg.hbm_budget / 2 ** 30),
-        roofline=dict(bound="mfma", achieved=ach, peak=FP64_PEAK, unit="TFLOP/s",
+        roofline=dict(bound="fp64-valu", achieved=ach, peak=FP64_PEAK, unit="TFLOP/s",
                       frac=ach / FP64_PEAK, traffic=SERVE_FOLD_TRAFFIC,
-                      traffic_source="rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE of this workload "
-                                     "(mean bytes per launch, copy-calibrated)",
+                      traffic_source=SERVE_FOLD_TRAFFIC_SOURCE,
                       kernel="tlsfold_kernel<true> (serving step)", launches=nlaunch,
                       kernel_ms=kms, trials_per_launch=ktr / max(nlaunch, 1),
                       evaluated_trials_per_launch=evaluated / max(nlaunch, 1),
                       flops_per_trial=TRIAL_FLOPS),
-        cpu_baseline=dict(value=cpu_lcs, unit="light curves/s", cores=1, kind="port",
-                          trial_evals_per_s=cpu_tps,
-                          sample=f"{done} of {len(r0.periods)} periods of one n={len(r0.t)} "
-                                 f"request (C oracle, direct window sums) + scipy medfilt, "
-                                 f"scaled by trial evals to the trace's mean request ({model})"),
+        cpu_baseline=dict(value=lcs(rates["c_allcores_curves"]), unit="light curves/s",
+                          cores=rates["cores"], kind="port",
+                          trial_evals_per_s=rates["c_allcores"],
+                          one_core=dict(light_curves_per_s=lcs(rates["c_1core_curves"]),
+                                        trial_evals_per_s=rates["c_1core"]),
+                          numpy_tls_py=dict(one_core_trial_evals_per_s=rates["numpy_1core"],
+                                            all_cores_trial_evals_per_s=rates["numpy_allcores"]),
+                          sample=f"random periods of one n={len(r0.t)} request "
+                                 f"({len(r0.periods)} periods), about 8 s per leg: the C oracle "
+                                 f"(direct window sums) on {rates['cores']} processes and on 1 "
+                                 f"core, and the numpy restatement of tls.py's statistic "
+                                 f"(vectorised over epochs); curves/s = periods/s over the grid, "
+                                 f"plus scipy medfilt, scaled by trial evaluations to the "
+                                 f"trace's mean request ({model})"),
     )
+
+
+# ----------------------------------------------------------------------------
+# open-loop serving: a Poisson stream for a fixed duration at several rates
+# ----------------------------------------------------------------------------
+OPENLOOP_RATES = tuple(float(v) for v in os.environ.get(
+    "GTRAP_OPENLOOP_RATES", "60,120,180,2000").split(","))
+OPENLOOP_SECONDS = float(os.environ.get("GTRAP_OPENLOOP_SECONDS", "10"))
+
+
+def bench_openloop(steps: int, warmup: int):
+    """Arrivals at each rate for OPENLOOP_SECONDS (default 10 s), served until
+    drained: queueing latency p50/p99 from arrival to result, the completion
+    rate, and the highest rate whose p99 stays below 2 s."""
+    import torch
+    from gtrap_amd import synth
+    from gtrap_amd.serve import ServerConfig, TransitServer
+    cfg = ServerConfig.tuned()
+    srv = TransitServer(cfg)
+    warm, _ = synth.serving_trace(40, seed=3000)
+    srv.run(warm, realtime=False)
+    torch.cuda.synchronize()
+    rows = []
+    for lam in OPENLOOP_RATES:
+        reqs, truth = synth.serving_stream(lam, OPENLOOP_SECONDS, seed=2004 + int(lam))
+        log(f"openloop: {len(reqs)} requests at {lam:g} req/s over {OPENLOOP_SECONDS:g} s")
+        t0 = time.perf_counter()
+        res = srv.run(reqs, t0=t0, realtime=True)
+        wall = time.perf_counter() - t0
+        lat = np.array([r.latency for r in res])
+        rows.append(dict(rate=lam, requests=len(res), wall_s=wall,
+                         completion_rate=len(res) / wall,
+                         p50_ms=float(np.percentile(lat, 50) * 1e3),
+                         p99_ms=float(np.percentile(lat, 99) * 1e3),
+                         max_ms=float(lat.max() * 1e3)))
+        log(f"  {rows[-1]}")
+    ok = [r["rate"] for r in rows if r["p99_ms"] < 2000.0]
+    top = rows[-1]
+    return dict(metric="open-loop serving: queueing latency p50/p99 and completion rate",
+                value=top["completion_rate"], unit="light curves/s",
+                ms_per_step=top["wall_s"] * 1e3, dtype="f64", rates=rows,
+                sustained_rate=max(ok) if ok else None,
+                config=dict(workload=f"open-loop Poisson streams of TLS+detrend requests for "
+                                     f"{OPENLOOP_SECONDS:g} s at "
+                                     f"{', '.join(f'{r:g}' for r in OPENLOOP_RATES)} req/s, "
+                                     f"served until drained", seconds=OPENLOOP_SECONDS),
+                roofline=None, cpu_baseline=None)
+
+
+# ----------------------------------------------------------------------------
+# single-curve workload: 4096 samples, 1000 periods 0.5-20 d, 10 W x 10 L
+# ----------------------------------------------------------------------------
+def bench_tls1(steps: int, warmup: int):
+    import torch
+    from gtrap_amd import synth, tlsfold
+    d = synth.tls_single_curve()
+    t, x, per, widths, nl = d["t"], d["flux"], d["periods"], d["widths"], d["nl"]
+    batch = tlsfold.pack([(t, x)], [per])
+    for _ in range(max(1, warmup)):
+        tlsfold.tlsfold_device(batch, widths, nl, synth.CADENCE)
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    reps = max(1, steps) * 20
+    t0 = time.perf_counter()
+    a.record()
+    for _ in range(reps):
+        out = tlsfold.tlsfold_device(batch, widths, nl, synth.CADENCE)
+    b.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kms = a.elapsed_time(b) / reps
+    sn = out["sn"].cpu().numpy()
+    from oracle import tlsfold as ot
+    best, bval, sde = ot.tlsstats(sn)
+    trials = int(np.sum(np.ceil(per / synth.CADENCE))) * len(widths) * nl
+    evaluated = trials * (nl - 1) / nl
+    ach = TRIAL_FLOPS * evaluated / (kms * 1e-3) / 1e12
+    log(f"tls1: {kms:.3f} ms per curve; best period {per[best]:.4f} d (injected "
+        f"{d['truth']['P']} d), S/N {bval:.3f}, SDE {sde:.2f}")
+    rates = cpu_trial_rates(t, x, per, widths, nl, synth.CADENCE, cap=6.0)
+    return dict(metric="light curves/s and trial evals/s (plain TLS, one curve)",
+                value=1e3 / kms, unit="light curves/s", ms_per_step=wall / reps * 1e3,
+                dtype="f64", trial_evals_per_s=trials / (kms * 1e-3),
+                detection=dict(best_period=float(per[best]), injected_period=d["truth"]["P"],
+                               recovered=bool(synth.recovered(per[best], d["truth"])),
+                               sn=float(bval), sde=float(sde)),
+                config=dict(workload="plain TLS: 1 curve x 4096 samples at 0.0204 d, trapezoid "
+                                     "depth 2e-3 at P 3.7 d, 1000 periods 0.5-20 d, 10 W x 10 L",
+                            n=len(t), periods=len(per)),
+                roofline=dict(bound="fp64-valu", achieved=ach, peak=FP64_PEAK, unit="TFLOP/s",
+                              frac=ach / FP64_PEAK, traffic=None, kernel="tlsfold_kernel<false>",
+                              kernel_ms=kms, flops_per_trial=TRIAL_FLOPS),
+                cpu_baseline=dict(value=rates["c_allcores_curves"], unit="light curves/s",
+                                  cores=rates["cores"], kind="port",
+                                  trial_evals_per_s=rates["c_allcores"],
+                                  one_core=rates["c_1core_curves"],
+                                  numpy_tls_py=dict(one_core=rates["numpy_1core_curves"],
+                                                    all_cores=rates["numpy_allcores_curves"]),
+                                  sample="random periods of the grid, about 6 s per leg: the C "
+                                         "oracle on a process pool and on 1 core, and the "
+                                         "numpy restatement of tls.py likewise (light "
+                                         "curves/s = periods/s over the 1000-period grid)"))
 
 
 # ----------------------------------------------------------------------------
@@ -540,7 +674,7 @@ def bench_tlsfit(steps: int, warmup: int):
 
 WORKLOADS = {"gfilter": bench_gfilter, "geebls": bench_geebls, "serve": bench_serve,
              "copy": bench_copy, "precision": bench_precision, "gtls": bench_gtls,
-             "tlsfit": bench_tlsfit}
+             "tlsfit": bench_tlsfit, "tls1": bench_tls1, "openloop": bench_openloop}
 
 
 def main(argv=None):

@@ -217,6 +217,50 @@ def serving_trace(count: int = 200, seed: int = 1004, rate: float = 2000.0,
     return out, truth
 
 
+def serving_stream(rate: float, duration: float, seed: int = 2004, nlo: int = 2048,
+                   nhi: int = 70000):
+    """Open-loop stream: Poisson arrivals at ``rate`` req/s over ``duration``
+    seconds, the requests of :func:`serving_trace` (same length, flux, transit
+    and period-grid distributions), generated in float32 to keep a long stream
+    cheap.  Returns (requests, truths) sorted by arrival (seconds from 0)."""
+    from .serve import Request
+    from .tlsfold import period_grid
+    rng = np.random.default_rng(seed)
+    arr = []
+    tt = 0.0
+    while True:
+        tt += rng.exponential(1.0 / rate)
+        if tt >= duration:
+            break
+        arr.append(tt)
+    out, truth = [], []
+    for i, a in enumerate(arr):
+        n = int(round(np.exp(rng.uniform(np.log(nlo), np.log(nhi)))))
+        t = np.arange(n, dtype=np.float64) * CADENCE
+        base = n * CADENCE
+        flux = 1.0 + 2e-3 * np.sin(2 * np.pi * t / rng.uniform(5, 50) + rng.uniform(0, 6.3))
+        flux += 1e-3 * rng.standard_normal(n, dtype=np.float32)
+        tr = None
+        if rng.random() < 0.3:
+            P = rng.uniform(0.5, base / 3)
+            t14 = rng.uniform(0.05, 0.3)
+            tr = dict(P=P, epoch=rng.uniform(0, P), depth=rng.uniform(1e-3, 5e-3), t14=t14)
+            flux -= trapezoid_dip(t, tr["epoch"], P, tr["depth"], t14, t14 / 10)
+        out.append(Request(i, t.astype(np.float32), flux.astype(np.float32),
+                           period_grid(base, n=n), arrival=float(a)))
+        truth.append(tr)
+    return out, truth
+
+
+def recovered(best_period: float, truth, tol: float = 0.01,
+              harmonics=(1.0 / 3, 0.5, 1.0, 2.0, 3.0)) -> bool:
+    """Whether a reported period is within ``tol`` (relative) of the injected
+    period or one of its harmonics."""
+    if truth is None or not np.isfinite(best_period):
+        return False
+    return any(abs(best_period - k * truth["P"]) <= tol * k * truth["P"] for k in harmonics)
+
+
 def trial_count(req, nw: int = 10, nl: int = 10, dtep: float = CADENCE) -> int:
     """(epoch, W, L) trial evaluations of one request."""
     return int(np.sum(np.ceil(np.asarray(req.periods) / dtep)) * nw * nl)

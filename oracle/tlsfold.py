@@ -156,3 +156,84 @@ def tlsstats(sn):
         var = (fin * fin).sum() / len(fin) - mean * mean if len(fin) else np.nan
         sde = (bval - mean) / np.sqrt(var)
     return best, bval, float(sde)
+
+
+def period_numpy(t, x, P, widths, nl, dtep):
+    """Best trial of one period with numpy, tls.py's statistic vectorised over
+    the epochs (the CPU path the benchmark times beside the GPU): every
+    usable sample is folded (:func:`fold_offsets`' rule), each epoch gathers
+    the samples within the widest half window (exact offsets decide every
+    region), and get_chi2Hmax (tls.py:44-79) runs on (epoch, sample) arrays
+    for each (W, L).  Returns (S/N, w, m, e, H) of the first maximum in
+    (W, L, epoch) order, NaN skipped (tls.py:100-102); (-inf, -1, -1, -1, NaN)
+    when no trial exists."""
+    t = np.asarray(t)
+    x = np.asarray(x)
+    ok = (t >= 0) & np.isfinite(x)
+    u = np.fmod(t[ok].astype(np.float64), P)
+    xv = x[ok].astype(np.float64)
+    o = np.argsort(u, kind="stable")
+    u, xv = u[o], xv[o]
+    k = len(u)
+    nE = int(np.ceil(P / dtep))
+    t0 = np.arange(nE) * dtep
+    hw = 0.5 * float(np.max(widths)) + 1e-9
+    if k == 0:
+        return -np.inf, -1, -1, -1, np.nan
+    if 2.0 * hw < P:
+        ue = np.concatenate([u - P, u, u + P])
+        src = np.tile(np.arange(k), 3)
+        lo = np.searchsorted(ue, t0 - hw, "left")
+        hi = np.searchsorted(ue, t0 + hw, "right")
+        K = max(1, int((hi - lo).max()))
+        idx = lo[:, None] + np.arange(K)[None, :]
+        valid = idx < hi[:, None]
+        idx = src[np.minimum(idx, 3 * k - 1)]
+    else:
+        idx = np.broadcast_to(np.arange(k), (nE, k))
+        valid = np.ones((nE, k), bool)
+    uu = u[idx]
+    c = t0[:, None]
+    d = uu - c
+    d = np.where(d >= 0.5 * P, (uu - P) - c, np.where(d < -0.5 * P, (uu + P) - c, d))
+    xx = np.where(valid, xv[idx], 0.0)
+    best = (-np.inf, -1, -1, -1, np.nan)
+    with np.errstate(all="ignore"):
+        for w, W in enumerate(widths):
+            if not W < P:
+                continue
+            close = valid & (d >= -W / 2) & (d <= W / 2)
+            mean = (xx * close).sum(1) / close.sum(1)
+            y = xx - mean[:, None]
+            for m in range(1, nl):
+                L = m * (0.45 * W / nl)
+                mA = valid & (d >= -W / 2) & (d < -W / 4 - L / 2)
+                mB = valid & (d >= -W / 4 - L / 2) & (d < -W / 4 + L / 2)
+                mC = valid & (d < W / 4 - L / 2) & (d >= -W / 4 + L / 2)
+                mD = valid & (d < W / 4 + L / 2) & (d >= W / 4 - L / 2)
+                mE = valid & (d < W / 2) & (d >= W / 4 + L / 2)
+                nA, nB, nC, nD, nE_ = (mk.sum(1) for mk in (mA, mB, mC, mD, mE))
+                n = nA + nB + nC + nD + nE_
+                xAEC = (y * mA).sum(1) + (y * mE).sum(1) + (y * mC).sum(1)
+                xBD = (y * mB).sum(1) + (y * mD).sum(1)
+                xtB = (y * d * mB).sum(1)
+                xtD = (y * d * mD).sum(1)
+                tB = (d * mB).sum(1)
+                tD = (d * mD).sum(1)
+                t2BD = (d * d * mB).sum(1) + (d * d * mD).sum(1)
+                denA = 4 * L * L * (nA - nC + nE_) + W * W * (nB + nD) + 8 * W * (tB - tD) + \
+                    16 * t2BD
+                numB = 8 * L * L * xAEC - 4 * W * L * xBD + 16 * L * (xtD - xtB)
+                H = -numB / denA
+                Hh = H / 2.0
+                res = ((((y + Hh[:, None]) ** 2) * (mA | mE)).sum(1)
+                       + (((y - Hh[:, None]) ** 2) * mC).sum(1)
+                       + (((y - (H / L)[:, None] * (d + W / 4.0)) ** 2) * mB).sum(1)
+                       + (((y + (H / L)[:, None] * (d - W / 4.0)) ** 2) * mD).sum(1))
+                sn = np.where(n > 3, -H / np.sqrt(res / (n - 3)), np.nan)
+                if np.all(np.isnan(sn)):
+                    continue
+                e = int(np.nanargmax(sn))
+                if sn[e] > best[0]:
+                    best = (float(sn[e]), w, m, e, float(H[e]))
+    return best

@@ -145,3 +145,16 @@ def test_too_few_samples_give_no_trial():
         t[:masked] = -1000.0
         r = ot.tlsfold(t, x, np.array([0.5, 0.03, 2.0]), np.linspace(0.04, 0.3, 10), 10, 0.0204)
         assert np.all(np.isneginf(r["sn"])) and np.all(r["w"] == -1), (n, masked, r)
+
+
+def test_numpy_period_search_equals_c_oracle():
+    """The numpy restatement (tls.py's masks, vectorised over epochs) and the C
+    oracle give the same best trial per period."""
+    t, x = _noisy(n=1200, seed=9)
+    widths = np.linspace(0.04, 0.3, 10)
+    per = np.array([0.5, 0.83, 1.7, 4.4, 9.9, 30.0])
+    c = ot.tlsfold(t, x, per, widths, 10, 0.0204)
+    for j, P in enumerate(per):
+        sn, w, m, e, H = ot.period_numpy(t, x, P, widths, 10, 0.0204)
+        assert (w, m, e) == (c["w"][j], c["l"][j], c["e"][j])
+        np.testing.assert_allclose([sn, H], [c["sn"][j], c["h"][j]], rtol=1e-9)
