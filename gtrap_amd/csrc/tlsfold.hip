@@ -30,6 +30,7 @@
 //     the boundary sums accumulated per (epoch, W, boundary).
 #include "common.h"
 #include <math.h>
+#include <mutex>
 
 namespace gtrap {
 
@@ -1182,6 +1183,71 @@ extern "C" int gtrap_tlsfold(double *snr, double *hbest, int32_t *best_w, int32_
   hipLaunchKernelGGL(gtrap::tlsfold_kernel<false>, dim3((unsigned)g), dim3(gtrap::kTfThreads),
                      dyn, st, A);
   return gtrap::check_launch("tlsfold_kernel");
+}
+
+// ---- the same search with no allocation: a static device workspace --------
+//
+// For callers that cannot allocate (code under HIP graph capture, tight
+// serving loops, harnesses that forbid hipMalloc): the work ticket and the
+// per-workgroup slots live in one file-scope device array of
+// kTfStaticBytes per GPU, held from the library's load on that GPU until the
+// process ends.  Host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device (resolved once per
+// device); the array's name is a host-side stand-in, never passed to a kernel.
+namespace gtrap {
+constexpr int64_t kTfStaticBytes = 64ll << 20;
+constexpr int kTfMaxDevices = 64;
+}  // namespace gtrap
+
+static __device__ __align__(256) unsigned char scratch[gtrap::kTfStaticBytes];
+
+namespace gtrap {
+static void *g_scratch_ptr[kTfMaxDevices];
+static hipError_t g_scratch_err[kTfMaxDevices];
+static std::once_flag g_scratch_once[kTfMaxDevices];
+
+// Device address of this GPU's scratch array (hipGetSymbolAddress, once per
+// device, with that device current).
+static int tf_scratch_ptr(void **out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kTfMaxDevices) {
+    set_error("gtrap_tlsfold_static: no current device (or device index >= %d)",
+              kTfMaxDevices);
+    return GTRAP_ELAUNCH;
+  }
+  std::call_once(g_scratch_once[dev], [dev]() {
+    g_scratch_err[dev] = hipGetSymbolAddress(&g_scratch_ptr[dev], HIP_SYMBOL(scratch));
+  });
+  if (g_scratch_err[dev] != hipSuccess || !g_scratch_ptr[dev]) {
+    set_error("gtrap_tlsfold_static: hipGetSymbolAddress failed on device %d: %s", dev,
+              hipGetErrorString(g_scratch_err[dev]));
+    return GTRAP_ELAUNCH;
+  }
+  *out = g_scratch_ptr[dev];
+  return GTRAP_OK;
+}
+}  // namespace gtrap
+
+extern "C" int gtrap_tlsfold_static(double *snr, double *hbest, int32_t *best_w,
+                                    int32_t *best_l, int32_t *best_e, const float *t,
+                                    const float *x, const int64_t *off, const int32_t *len,
+                                    int32_t nmax, const double *periods, const int64_t *poff,
+                                    int32_t ncurves, int64_t nptot, const double *widths,
+                                    int32_t nw, int32_t nl, double dtep, int32_t nblocks,
+                                    void *stream) {
+  GTRAP_REQUIRE(nmax >= 1 && nblocks >= 1, "gtrap_tlsfold_static: bad nmax or nblocks");
+  const int64_t slot = tf_slot_bytes(nmax);
+  GTRAP_REQUIRE(256 + slot <= gtrap::kTfStaticBytes,
+                "gtrap_tlsfold_static: nmax = %d needs a %lld-byte workgroup slot; the static "
+                "workspace holds %lld bytes (use gtrap_tlsfold with a caller workspace)",
+                nmax, (long long)slot, (long long)gtrap::kTfStaticBytes);
+  void *ws = nullptr;
+  if (const int rc = gtrap::tf_scratch_ptr(&ws)) return rc;
+  const int64_t fit = (gtrap::kTfStaticBytes - 256) / slot;
+  const int32_t nb = (int64_t)nblocks < fit ? nblocks : (int32_t)fit;
+  return gtrap_tlsfold(snr, hbest, best_w, best_l, best_e, t, x, off, len, nmax, periods, poff,
+                       ncurves, nptot, widths, nw, nl, dtep, ws, gtrap::kTfStaticBytes, nb,
+                       stream);
 }
 
 extern "C" int gtrap_tlsfold_items(double *snr, double *hbest, int32_t *best_w, int32_t *best_l,

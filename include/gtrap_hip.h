@@ -311,6 +311,28 @@ int gtrap_tlsfold(double *snr, double *hbest, int32_t *best_w, int32_t *best_l,
                   void *ws, int64_t ws_bytes, int32_t nblocks, void *stream);
 
 /*
+ * gtrap_tlsfold with no allocation and no workspace argument.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The work ticket and the per-workgroup
+ * slots live in a static 64 MiB device array per GPU (held from the
+ * library's load until the process ends); it calls no hipMalloc/hipFree.
+ * nblocks is clamped to the slots that fit: 64 MiB / gtrap_tlsfold_ws_bytes
+ * (nmax, 1), e.g. 53 workgroups for nmax = 70000, 907 for nmax = 4096; an
+ * nmax whose single slot does not fit (about 3.7 million samples) is
+ * rejected with GTRAP_EINVAL.  Results are identical to gtrap_tlsfold's.
+ * Calls that overlap on one GPU (other streams or host threads) share the
+ * array and race: serialise them.  The ticket is reset by a kernel on the
+ * call's stream before every launch.
+ */
+int gtrap_tlsfold_static(double *snr, double *hbest, int32_t *best_w, int32_t *best_l,
+                         int32_t *best_e, const float *t, const float *x, const int64_t *off,
+                         const int32_t *len, int32_t nmax, const double *periods,
+                         const int64_t *poff, int32_t ncurves, int64_t nptot,
+                         const double *widths, int32_t nw, int32_t nl, double dtep,
+                         int32_t nblocks, void *stream);
+
+/*
  * Item-list form of gtrap_tlsfold for the serving loop (no reference
  * counterpart).  Work item k scores period item_pidx[k] (index into periods
  * and the outputs) of curve slot item_curve[k] (off/len tables); the item
