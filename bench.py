@@ -352,7 +352,8 @@ def bench_openloop(steps: int, warmup: int):
         reqs, truth = synth.serving_stream(lam, OPENLOOP_SECONDS, seed=2004 + int(lam))
         log(f"openloop: {len(reqs)} requests at {lam:g} req/s over {OPENLOOP_SECONDS:g} s")
         t0 = time.perf_counter()
-        res = srv.run(reqs, t0=t0, realtime=True)
+        res = srv.run(reqs, t0=t0, realtime=True,
+                      progress=lambda d, q: log(f"  {d} done, {q} queued"))
         wall = time.perf_counter() - t0
         lat = np.array([r.latency for r in res])
         rows.append(dict(rate=lam, requests=len(res), wall_s=wall,

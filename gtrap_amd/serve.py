@@ -738,16 +738,23 @@ class TransitServer:
     def busy(self) -> bool:
         return bool(self.queue or self.slots)
 
-    def run(self, requests, t0: float | None = None, realtime: bool = True) -> list[Result]:
+    def run(self, requests, t0: float | None = None, realtime: bool = True,
+            progress=None, every: float = 5.0) -> list[Result]:
         """Serve a trace.  ``arrival`` of each request is seconds after t0; with
-        realtime=False every request is available immediately."""
+        realtime=False every request is available immediately.  ``progress``,
+        if given, is called with (results so far, requests queued) at most
+        every ``every`` seconds."""
         t0 = time.perf_counter() if t0 is None else t0
         reqs = sorted(requests, key=lambda r: r.arrival)
         for r in reqs:
             r.arrival = t0 + (r.arrival if realtime else 0.0)
             self.submit(r)
         results = []
+        last = time.perf_counter()
         while self.busy():
+            if progress is not None and time.perf_counter() - last > every:
+                last = time.perf_counter()
+                progress(len(results), len(self.queue))
             now = time.perf_counter()
             pending = any(sl.next_item < len(sl.req.periods) for s, sl in self.slots.items()
                           if s not in self.dead)

@@ -84,3 +84,44 @@ def test_geebls_f64_mode_is_the_float64_variant(gpu):
                              nthread=g["nthread"])
     np.testing.assert_array_equal(sr.cpu().numpy().reshape(nq, nf), want.cpu().numpy())
     np.testing.assert_array_equal(p1.cpu().numpy().reshape(nq, nf), w1.cpu().numpy())
+
+
+def test_long_baseline_full_length_curves(gpu):
+    """The precision workload at its own scale on a stated subset: 32 curves of
+    262144 samples at 2-minute cadence (364 d, t*f up to ~50 cycles), 256
+    frequencies around the transit period.  The build's default arithmetic
+    equals gtrap_geebls bit for bit, the float64 variant equals the float64
+    oracle on sampled cells, and float64-fold variants stay within 1e-5 of
+    it while the float32 fold (the reference's) shows its larger error."""
+    import torch
+    from gtrap_amd import geebls, precision, synth
+    d = synth.precision_batch(nq=32, n=262144, nf=256, seed=15)
+    g = d["grid"]
+    inp = precision.Inputs(d)
+    n, nq, nf = inp.n, inp.nq, g["nf"]
+    assert n == 262144 and d["t64"][-1] > 363.0
+    sr, p1, p2 = precision.run_variant(inp, "f32", "exact32", "f32")
+    blsp = torch.empty(nq * nf, dtype=torch.float32, device="cuda")
+    q1 = torch.empty(nq * nf, dtype=torch.int32, device="cuda")
+    q2 = torch.empty(nq * nf, dtype=torch.int32, device="cuda")
+    geebls.geebls_device(blsp, q1, q2, inp.lc, inp.tu, inp.offset, inp.ntrue, n, nq, nf,
+                         g["nb"], g["kma"], g["kmi"], g["kkmi"], g["fmin"], g["df"],
+                         nthread=g["nthread"])
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(sr.cpu().numpy(),
+                                  blsp.cpu().numpy().reshape(nq, nf).astype(np.float64))
+    np.testing.assert_array_equal(p1.cpu().numpy(), q1.cpu().numpy().reshape(nq, nf))
+    truth, _, _ = precision.run_variant(inp, *precision.TRUTH)
+    truth = truth.cpu().numpy()
+    rng = np.random.default_rng(5)
+    cells = [(int(rng.integers(0, nq)), int(rng.integers(0, nf))) for _ in range(10)]
+    tu = np.broadcast_to(d["t64"].astype(np.float32)[:, None], d["lc"].shape)
+    want = op.bls64_cells(d["lc"], d["t64"], tu, d["offset"], d["ntrue"], g["nb"], g["kma"],
+                          g["kmi"], g["kkmi"], g["fmin"], g["df"], g["nthread"], cells)
+    np.testing.assert_allclose([truth[q, jf] for q, jf in cells], want, rtol=1e-9)
+    res = precision.study(d)
+    print({k: (v["sr_rel_median"], v["sr_rel_max"]) for k, v in res.items()
+           if "yardstick" not in k}, flush=True)
+    for name in ("f32/exact32/f64", "f32/twosum32/f64", "f32/atomic32/f64"):
+        assert res[name]["sr_rel_max"] < 1e-5, (name, res[name])
+    assert res["f32/f64/f32"]["sr_rel_max"] > 10 * res["f32/exact32/f64"]["sr_rel_max"]
