@@ -513,21 +513,17 @@ def bench_copy(steps: int, warmup: int):
 GTLS_NQ = 4
 
 
-def gtls_lds_bytes_per_trial(widths, nsc: int, nl: int, deltat: float) -> float:
-    """LDS bytes the sequential scan of one (epoch, W, L) trial reads (4 B
-    floats): the region scans read time and flux over the window
-    (2 W/deltat), the residual pass reads them again (2 W/deltat); the per-W
-    mean pass (2 nsc) and skip scan up to -W/2 (nsc/2 - W/2/deltat times) are
-    shared by the nl trials.  Mean over the width grid."""
-    w = np.asarray(widths, np.float64) / deltat
-    skip = np.maximum(0.0, nsc / 2.0 - w / 2.0)
-    return float(np.mean(4.0 * (4.0 * w + (2.0 * nsc + skip) / nl)))
+# FP64 operations of one (epoch, W, L) trial of the fast gtls kernel from the
+# tile prefix sums: region sums (prefix differences, the B/D moments moved to
+# epoch coordinates) 33, region-A index quirk 5, centred sums 17, denA 16,
+# numB 11, H and its halves 4, the expanded residual 37, S/N 4; +, -, *, /,
+# sqrt one each
+GTLS_TRIAL_FLOPS = 127
 
 
 def bench_gtls(steps: int, warmup: int):
     import torch
     from gtrap_amd import gtls
-    from oracle import gtls as og
     n, nq, nw, nl = 72000, GTLS_NQ, 1024, 20
     wmin, wmax = 1.0, 6.0
     rng = np.random.default_rng(1006)
@@ -546,47 +542,55 @@ def bench_gtls(steps: int, warmup: int):
     outs = [torch.empty(n * nq, dtype=torch.float32, device="cuda") for _ in range(5)]
     log(f"gtls: {nq} curves x {n} samples, nw {nw}, nl {nl}, nsc {nsc}")
 
-    def run():
-        gtls.gtls_device(*outs, dl, dt, n, nq, nw, nl, nsc, wmax, dw, deltat)
+    def timed(arith, reps):
+        def run():
+            gtls.gtls_device(*outs, dl, dt, n, nq, nw, nl, nsc, wmax, dw, deltat, arith=arith)
+        for _ in range(max(1, warmup)):
+            run()
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        a.record()
+        for _ in range(reps):
+            run()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps, (time.perf_counter() - t0) / reps
 
-    for _ in range(warmup):
-        run()
-    torch.cuda.synchronize()
-    a = torch.cuda.Event(enable_timing=True)
-    b = torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    a.record()
-    for _ in range(steps):
-        run()
-    b.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kms = a.elapsed_time(b) / steps
+    kms, wall = timed("fast", steps)
+    fast_out = [o.cpu().numpy() for o in outs]
+    pms, _ = timed("parity", 1)
+    par_out = [o.cpu().numpy() for o in outs]
     trials = float(n) * nq * nw * nl
-    widths = wmax - dw * np.arange(nw)
-    bpt = gtls_lds_bytes_per_trial(widths, nsc, nl, deltat)
-    ach = trials * bpt / (kms * 1e-3) / 1e12
-    log(f"gtls kernel {kms:.2f} ms; trials/s {trials / kms * 1e3:.3e}; {bpt:.0f} LDS B/trial")
-    # CPU: the C restatement over a bounded slice of epochs of one curve
+    ach = GTLS_TRIAL_FLOPS * trials / (kms * 1e-3) / 1e12
+    pos = (fast_out[0] > 0) & (par_out[0] > 0)
+    rel = np.abs(fast_out[0][pos] - par_out[0][pos]) / fast_out[0][pos]
+    log(f"gtls fast {kms:.2f} ms ({trials / kms * 1e3:.3e} trials/s), parity {pms:.2f} ms; "
+        f"fast vs float32 parity S/N rel. diff median {np.median(rel):.2e}")
+    # CPU: the float32 restatement over bounded slices of epochs
+    from oracle import timing
     cores, model = cpu_info()
-    k0, nk = n // 2, 600  # about 13 s of one core
-    c0 = time.perf_counter()
-    og.gtls(lc[:, :1], tu[:, :1], nw, nl, nsc, wmax, dw, deltat, ks=(k0, k0 + nk))
-    csec = time.perf_counter() - c0
-    cpu_lcs = 1.0 / (csec * n / nk)
+    r = timing.gtls_rates(lc, tu, nw, nl, nsc, wmax, dw, deltat, nk=300)
+    log(f"cpu gtls: 1 core {r['one_core']:.1f}, {r['cores']} processes {r['all_cores']:.1f} "
+        f"epochs/s")
     return dict(metric="light curves/s and (epoch, W, L) trials/s (gtls single-transit search)",
-                value=nq / (kms * 1e-3), unit="light curves/s", ms_per_step=wall / steps * 1e3,
-                dtype="f32", trials_per_s=trials / (kms * 1e-3),
+                value=nq / (kms * 1e-3), unit="light curves/s", ms_per_step=wall * 1e3,
+                dtype="f64", trials_per_s=trials / (kms * 1e-3),
+                parity=dict(value=nq / (pms * 1e-3), unit="light curves/s", kernel_ms=pms,
+                            dtype="f32", sn_rel_diff_median=float(np.median(rel)),
+                            sn_rel_diff_p99=float(np.percentile(rel, 99)),
+                            note="the reference's float32 sequential scans, bit-exact mode"),
                 config=dict(workload=f"gtls normal mode, {nq} curves x {n} samples at 0.0204 d "
                                      f"(2% masked), nw {nw} widths 1-6 d, nl {nl}, nsc {nsc}",
                             n=n, nq=nq, nw=nw, nl=nl, nsc=nsc),
-                roofline=dict(bound="lds", achieved=ach, peak=LDS_PEAK_TBS, unit="TB/s",
-                              frac=ach / LDS_PEAK_TBS, traffic=None, kernel="gtls_kernel",
-                              kernel_ms=kms, lds_bytes_per_trial=bpt),
-                cpu_baseline=dict(value=cpu_lcs, unit="light curves/s", cores=1, kind="port",
-                                  trials_per_s=cpu_lcs * n * nw * nl,
-                                  sample=f"{nk} epochs of one curve with the C oracle, "
-                                         f"scaled to {n} epochs ({model})"))
+                roofline=dict(bound="fp64-valu", achieved=ach, peak=FP64_PEAK, unit="TFLOP/s",
+                              frac=ach / FP64_PEAK, traffic=None, kernel="gtls_fast_kernel",
+                              kernel_ms=kms, flops_per_trial=GTLS_TRIAL_FLOPS),
+                cpu_baseline=dict(value=r["all_cores"] / n, unit="light curves/s",
+                                  cores=r["cores"], kind="port", one_core=r["one_core"] / n,
+                                  sample=f"300 epochs per process of one curve with the float32 "
+                                         f"C restatement, scaled to {n} epochs ({model})"))
 
 
 # ----------------------------------------------------------------------------

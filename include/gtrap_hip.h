@@ -247,6 +247,7 @@ int gtrap_blsstats(const float *blsp, const uint32_t *phase1, const uint32_t *ph
 /* ---- single-transit-epoch trapezoid least squares (gtls) ------------------ */
 #define GTRAP_GTLS_NORMAL   0  /* include/gtls_simple.h   */
 #define GTRAP_GTLS_ABSOLUTE 1  /* include/gtls_absolute.h */
+#define GTRAP_GTLS_PARITY   2  /* or-ed into mode: the float32 sequential scans */
 
 /*
  * Replaces the gtls kernel of include/gtls_simple.h:1-406 and
@@ -255,6 +256,19 @@ int gtrap_blsstats(const float *blsp, const uint32_t *phase1, const uint32_t *ph
  * (the reference's blockDim.x, a power of two), nl ingress steps, scoop nsc.
  * Outputs (interleaved, index k*nq + q): tlssn, tlsw, tlst0, tlsl, tlshmax.
  * t0min and dt are accepted and unused, as in the reference.
+ *
+ * Two arithmetics, the same region decisions (shifted time fl32(T - deltat*k)
+ * against the float32 boundaries, the region-A flux index quirk, the float32
+ * fill factor):
+ *   default: float64 sums, H, residual and S/N on the exact shifted time,
+ *     O(log nsc) per (epoch, W, L) trial from tile prefix sums (nsc up to about
+ *     2700); S/N is rounded to float32 at the end;
+ *   mode | GTRAP_GTLS_PARITY: the reference's float32 sequential scans in its
+ *     operation order, bit-identical to a float32 restatement (nsc up to
+ *     19200).
+ * The float32 arithmetic itself differs from the float64 one by a median of
+ * a few 1e-6 and up to about 1e-2 relative in S/N (cancellation of
+ * x - mean and of denA in float32).
  */
 int gtrap_gtls(float *tlssn, float *tlsw, float *tlst0, float *tlsl, float *tlshmax,
                const float *lc, const float *tu, int32_t nt, int32_t nq, int32_t nw,

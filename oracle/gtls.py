@@ -27,3 +27,24 @@ def gtls(lc, tu, nw, nl, nsc, wmax, dw, deltat, mode="normal", ks=None):
       np.float32(dw), np.float32(deltat), 1 if mode == "absolute" else 0, k0, k1)
     names = ("tlssn", "tlsw", "tlst0", "tlsl", "tlshmax")
     return {k: o.reshape(nt, nq) for k, o in zip(names, outs)}
+
+
+def gtls64(lc, tu, nw, nl, nsc, wmax, dw, deltat, mode="normal", ks=None):
+    """gtls with the reference's float32 region decisions and float64
+    arithmetic (``orc_gtls64``); also returns the float64 best S/N as
+    ``sn64``."""
+    lc = np.ascontiguousarray(lc, np.float32)
+    tu = np.ascontiguousarray(tu, np.float32)
+    nt, nq = lc.shape
+    outs = [np.zeros(nt * nq, np.float32) for _ in range(5)]
+    sn64 = np.zeros(nt * nq, np.float64)
+    f = lib().orc_gtls64
+    f.restype = None
+    f.argtypes = [C.c_void_p] * 8 + [C.c_int] * 5 + [C.c_float] * 3 + [C.c_int] * 3
+    k0, k1 = (0, nt) if ks is None else ks
+    f(*(fptr(o) for o in outs), fptr(sn64), fptr(lc), fptr(tu), nt, nq, nw, nl, nsc,
+      np.float32(wmax), np.float32(dw), np.float32(deltat), 1 if mode == "absolute" else 0, k0, k1)
+    names = ("tlssn", "tlsw", "tlst0", "tlsl", "tlshmax")
+    out = {k: o.reshape(nt, nq) for k, o in zip(names, outs)}
+    out["sn64"] = sn64.reshape(nt, nq)
+    return out

@@ -528,3 +528,150 @@ void orc_gtls(float *tlssn, float *tlsw, float *tlst0, float *tlsl, float *tlshm
   }
   free(cache);
 }
+
+/* ------------------------------------------------------------------------ */
+/* gtls in float64 (the arithmetic of the build's fast gtls mode).  The      */
+/* region membership is decided exactly as include/gtls_simple.h decides it: */
+/* shifted times ct = fl32(T - deltat*k) (one rounding, a fused multiply-    */
+/* add) against float32 boundaries tStart, tAB, tBC, tCD, tDE, tEnd, the    */
+/* sequential scans with their breaks (:199-305), the region-A flux index   */
+/* quirk (:214-221), the float32 fill factor (:308-311), the mean over the  */
+/* closed window (:139-152).  Every sum, H (:314-316), the residual         */
+/* (:324-357) and S/N (:359-369) are float64, with the exact shifted time   */
+/* tau = T - deltat*k (float64, no rounding) in the sums.  Ties resolve to  */
+/* the first L and then the lowest W index.                                 */
+/* ------------------------------------------------------------------------ */
+static void gtls_thread64(const float *x, const float *ct, const double *tau, int nsc, int nl,
+                          float W, float deltat, int mode, double *res_max_out,
+                          float *Lmax_out, double *Hmax_out) {
+  const float dl = mode ? (float)((double)W / 4.0 / (double)(float)nl)
+                        : (float)((double)W * 0.45 / (double)(float)nl);
+  const float tEnd = (float)(0.5 * (double)W), tStart = (float)(-0.5 * (double)W);
+  double msum = 0.0;
+  int mcnt = 0;
+  for (int m = 0; m < nsc; ++m)
+    if (ct[m] >= tStart && ct[m] <= tEnd) { msum += (double)x[m]; mcnt++; }
+  const double mean = mcnt > 0 ? msum / (double)mcnt : 0.0;
+  int nSx = 0;
+  for (int i = 0; i < nsc; ++i) { if (ct[i] > tStart) break; nSx++; }
+  double res_max = 0.0, Hmaxmax = 0.0;
+  float Lmax = 0.0f;
+  const double dW = (double)W;
+  for (int mm = 0; mm < nl; ++mm) {
+    const float L = (mode ? (float)(mm + 1) : (float)mm) * dl;
+    const double dL = (double)L;
+    double xAEC = 0, xBD = 0, xtB = 0, xtD = 0, t2BD = 0, tB = 0, tD = 0;
+    int nA = 0, nB = 0, nC = 0, nD = 0, nE = 0;
+    int nAx = 0, nBx = 0, nCx = 0, nDx = 0, nEx = 0;
+    const float tAB = (float)((double)(-W) * 0.25 - (double)L * 0.5);
+    const float tBC = (float)((double)(-W) * 0.25 + (double)L * 0.5);
+    const float tCD = -tBC, tDE = -tAB;
+    for (int i = 0; i < nsc - nSx; ++i) {
+      if (ct[i + nSx] > tAB) break;
+      if (ct[i + nSx] >= tStart) { nA++; xAEC += (double)x[i] - mean; }  /* A.1 quirk */
+      nAx++;
+    }
+    for (int i = nSx + nAx; i < nsc; ++i) {
+      if (ct[i] > tBC) break;
+      if (ct[i] >= tStart) {
+        const double y = (double)x[i] - mean;
+        nB++; xBD += y; tB += tau[i]; xtB += y * tau[i]; t2BD += tau[i] * tau[i];
+      }
+      nBx++;
+    }
+    for (int i = nSx + nAx + nBx; i < nsc; ++i) {
+      if (ct[i] > tCD) break;
+      if (ct[i] >= tStart) { nC++; xAEC += (double)x[i] - mean; }
+      nCx++;
+    }
+    for (int i = nSx + nAx + nBx + nCx; i < nsc; ++i) {
+      if (ct[i] > tDE) break;
+      if (ct[i] >= tStart) {
+        const double y = (double)x[i] - mean;
+        nD++; xBD += y; tD += tau[i]; xtD += y * tau[i]; t2BD += tau[i] * tau[i];
+      }
+      nDx++;
+    }
+    for (int i = nSx + nAx + nBx + nCx + nDx; i < nsc; ++i) {
+      if (ct[i] > tEnd) break;
+      if (ct[i] >= tStart) { nE++; xAEC += (double)x[i] - mean; }
+      nEx++;
+    }
+    const int n = nA + nB + nC + nD + nE;
+    const float filfac = (float)n / (W / deltat);
+    if (!((double)filfac > G_FILFACCRIT)) continue;
+    const double denA = 4.0 * dL * dL * (double)(nA - nC + nE) + dW * dW * (double)(nB + nD) +
+                        8.0 * dW * (tB - tD) + 16.0 * t2BD;
+    const double numB = 8.0 * dL * dL * xAEC - 4.0 * dW * dL * xBD + 16.0 * dL * (xtD - xtB);
+    const double H = -numB / denA, Hh = H / 2.0, HL = H / dL;
+    double res = 0.0;
+    const int eA = nSx + nAx, eB = eA + nBx, eC = eB + nCx, eD = eC + nDx, eE = eD + nEx;
+    for (int i = nSx; i < eE; ++i) {
+      if (!(ct[i] >= tStart)) continue;
+      const double y = (double)x[i] - mean;
+      double e;
+      if (i < eA) e = y + Hh;
+      else if (i < eB) e = y - HL * (tau[i] + dW / 4.0);
+      else if (i < eC) e = y - Hh;
+      else if (i < eD) e = y + HL * (tau[i] - dW / 4.0);
+      else e = y + Hh;
+      res += e * e;
+    }
+    if (res > 0.0) {
+      const double hv = mode ? fabs(H) : H;
+      const double sn = hv / sqrt(res / (double)(n - 3));
+      if (sn > res_max) { res_max = sn; Lmax = L; Hmaxmax = H; }
+    }
+  }
+  *res_max_out = res_max;
+  *Lmax_out = Lmax;
+  *Hmax_out = Hmaxmax;
+}
+
+void orc_gtls64(float *tlssn, float *tlsw, float *tlst0, float *tlsl, float *tlshmax,
+                double *sn64, const float *lc, const float *tu, int nt, int nq, int nw, int nl,
+                int nsc, float wmax, float dw, float deltat, int mode, int k0, int k1) {
+  float *x = (float *)malloc(sizeof(float) * (size_t)(nsc > 0 ? nsc : 1));
+  float *ct = (float *)malloc(sizeof(float) * (size_t)(nsc > 0 ? nsc : 1));
+  double *tau = (double *)malloc(sizeof(double) * (size_t)(nsc > 0 ? nsc : 1));
+  const int nsch = nsc / 2;
+  for (int k = k0; k < k1; ++k) {
+    for (int q = 0; q < nq; ++q) {
+      for (int i = 0; i < nsc; ++i) {
+        float T = (float)G_FILLVAL, f = 0.0f;
+        const int s = k + i - nsch;
+        if (s >= 0 && s < nt) {
+          f = lc[(size_t)s * nq + q];
+          T = T + (float)((double)tu[(size_t)s * nq + q] - G_FILLVAL);
+        }
+        if (T >= 0.0f) {
+          ct[i] = fmaf(-deltat, (float)k, T);
+          tau[i] = (double)T - (double)deltat * (double)k;
+          x[i] = f;
+        } else {
+          ct[i] = -1000.0f;
+          tau[i] = 0.0;
+          x[i] = 0.0f;
+        }
+      }
+      double best = 0.0, bH = 0.0;
+      float bL = 0.0f, bW = 0.0f;
+      int have = 0;
+      for (int iw = 0; iw < nw; ++iw) {
+        const float W = fmaf(-dw, (float)iw, wmax);
+        double rm, Hm;
+        float Lm;
+        gtls_thread64(x, ct, tau, nsc, nl, W, deltat, mode, &rm, &Lm, &Hm);
+        if (!have || rm > best) { best = rm; bL = Lm; bH = Hm; bW = W; have = 1; }
+      }
+      const size_t o = (size_t)k * nq + q;
+      tlssn[o] = (float)best;
+      if (sn64) sn64[o] = best;
+      tlst0[o] = deltat * (float)k;
+      tlsw[o] = bW;
+      tlsl[o] = bL;
+      tlshmax[o] = (float)bH;
+    }
+  }
+  free(x); free(ct); free(tau);
+}

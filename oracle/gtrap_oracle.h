@@ -6,7 +6,6 @@
 /* gtrap/gfilter.py:13-85 -- bisection median with zero halo; mode 0 sub, 1 div. */
 void orc_gfilter_bisect(const float *in, float *out, int n, int nq, int r, int mode);
 
-#endif
 
 /* include/geebls.h:1-167 (simple=0) and gtrap/geebls_simple.py:17-175 (simple=1).
  * Bin sums are exact fixed-point sums with per-curve scale 2^-S (see bls_scale_exp),
@@ -36,3 +35,12 @@ double orc_tlsfold_trial(const float *t, const float *x, int n, double P, int e,
 void orc_gtls(float *tlssn, float *tlsw, float *tlst0, float *tlsl, float *tlshmax,
               const float *lc, const float *tu, int nt, int nq, int nw, int nl, int nsc,
               float wmax, float dw, float deltat, int mode, int k0, int k1);
+
+/* gtls with float32 region membership (as include/gtls_simple.h decides it)
+ * and float64 sums, H, residual and S/N on the exact shifted time (the build's
+ * fast gtls mode); sn64 (may be null) receives the float64 best S/N. */
+void orc_gtls64(float *tlssn, float *tlsw, float *tlst0, float *tlsl, float *tlshmax,
+                double *sn64, const float *lc, const float *tu, int nt, int nq, int nw, int nl,
+                int nsc, float wmax, float dw, float deltat, int mode, int k0, int k1);
+
+#endif /* GTRAP_ORACLE_H */

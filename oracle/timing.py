@@ -63,3 +63,36 @@ def trial_rates(t, x, periods, widths, nl, dtep, cap=8.0, max_procs=16, log=None
             log(f"cpu {kind}: 1 core {out[f'{kind}_1core']:.3e}, {cores} processes "
                 f"{out[f'{kind}_allcores']:.3e} trial evals/s")
     return out
+
+
+def gtls_epochs_worker(lc, tu, nw, nl, nsc, wmax, dw, deltat, k0, k1):
+    """The float32 gtls restatement over epochs [k0, k1) of the given curves;
+    returns seconds."""
+    for v in ("OMP_NUM_THREADS", "MKL_NUM_THREADS", "OPENBLAS_NUM_THREADS"):
+        os.environ[v] = "1"
+    from . import gtls as og
+    t1 = time.perf_counter()
+    og.gtls(lc, tu, nw, nl, nsc, wmax, dw, deltat, ks=(k0, k1))
+    return time.perf_counter() - t1
+
+
+def gtls_rates(lc, tu, nw, nl, nsc, wmax, dw, deltat, nk=400, max_procs=16):
+    """Epochs/s of the float32 gtls restatement on one core and on a process
+    pool (each process a different slice of nk epochs of curve 0)."""
+    import concurrent.futures as cf
+    import multiprocessing as mp
+    cores = max(1, min(max_procs, len(os.sched_getaffinity(0))))
+    nt = lc.shape[0]
+    c1 = lc[:, :1].copy()
+    t1_ = tu[:, :1].copy()
+    k0 = nt // 2
+    sec = gtls_epochs_worker(c1, t1_, nw, nl, nsc, wmax, dw, deltat, k0, k0 + nk)
+    starts = [int(v) for v in np.linspace(nsc, nt - nsc - nk, cores)]
+    t0 = time.perf_counter()
+    with cf.ProcessPoolExecutor(cores, mp_context=mp.get_context("spawn")) as ex:
+        futs = [ex.submit(gtls_epochs_worker, c1, t1_, nw, nl, nsc, wmax, dw, deltat, a, a + nk)
+                for a in starts]
+        for f in futs:
+            f.result(timeout=600)
+    wall = time.perf_counter() - t0
+    return dict(cores=cores, one_core=nk / sec, all_cores=cores * nk / wall)

@@ -114,3 +114,75 @@ def test_residual_identity_of_the_best_trial():
         np.testing.assert_allclose(res_implied, res_direct, rtol=2e-4)
         checked += 1
     assert checked > 30
+
+
+def test_float64_gtls_matches_tls_py_when_quirks_are_void():
+    """The float64 restatement equals tls.py's float64 statistic where the
+    scoop lies inside the window (no region-A index shift); the reference's
+    L is float32 (gtls_simple.h:69,167), so 1e-7."""
+    t, x = _series(dip=(100, 2, 0.05))
+    lc, tu = x[:, None], t[:, None]
+    nsc, nl, wmax = 9, 5, 9.0
+    out = og.gtls64(lc, tu, 1, nl, nsc, wmax, 0.0, 1.0)
+    for k in (50, 100, 150):
+        best = 0.0
+        for m in range(1, nl):
+            sn, H, n = ot.trial_numpy(t, x, 1e9, k, wmax, m, nl, 1.0)
+            best = max(best, -sn) if np.isfinite(sn) else best
+        np.testing.assert_allclose(out["sn64"][k, 0], best, rtol=1e-7)
+
+
+def test_float64_gtls_residual_identity():
+    """S/N = H/sqrt(res/(n-3)) of the float64 restatement implies the direct
+    residual against gen_trapzoid with the exact shifted time (1e-9)."""
+    from tests.test_oracle_tls_pins import gen_trapzoid
+    rng = np.random.default_rng(22)
+    nt = 600
+    t = (np.arange(nt) * 0.0204).astype(np.float32)
+    x = (1e-3 * rng.standard_normal(nt)).astype(np.float32)
+    x[(t > 5.0) & (t < 5.12)] += 3e-3
+    tu = t.copy()
+    tu[300:310] = -1000.0
+    nw, nl, wmax, dw, deltat = 8, 6, 0.3, 0.03, np.float32(0.0204)
+    nsc = int(wmax / deltat + 3)
+    out = og.gtls64(x[:, None], tu[:, None], nw, nl, nsc, wmax, dw, deltat)
+    checked = 0
+    for k in range(20, nt - 20, 7):
+        sn = float(out["sn64"][k, 0])
+        if sn <= 0:
+            continue
+        W, L, H = (float(out[n][k, 0]) for n in ("tlsw", "tlsl", "tlshmax"))
+        tau32, xx = _gtls_scoop(x, tu, k, nsc, deltat)
+        rows = np.arange(k - nsc // 2, k - nsc // 2 + nsc)
+        rows = rows[(rows >= 0) & (rows < nt)]
+        T = (np.float32(-1000.0) + (tu[rows] + np.float32(1000.0))).astype(np.float32)
+        tau = T[T >= 0].astype(np.float64) - float(deltat) * k
+        W32 = np.float32(W)
+        lo, hi = np.float32(-0.5 * W32), np.float32(0.5 * W32)
+        mean = xx[(tau32 >= lo) & (tau32 <= hi)].mean()
+        win = (tau32 > lo) & (tau32 <= hi)
+        r = xx[win] - mean - gen_trapzoid(tau[win], H, W, L, 0.0) * 1.0
+        res_direct = float(np.sum(r * r))
+        # tlshmax is H rounded to float32: recompute the implied residual from
+        # the float64 S/N with that H and compare at the float32 H's precision
+        res_implied = (win.sum() - 3) * (H / sn) ** 2
+        np.testing.assert_allclose(res_implied, res_direct, rtol=2e-6)
+        checked += 1
+    assert checked > 30
+
+
+def test_float32_arithmetic_error_against_float64():
+    """The float32 restatement (the reference's arithmetic) against the float64
+    one on flipped, near-unit flux: a median of a few 1e-6 in S/N, the same
+    best trials almost everywhere."""
+    rng = np.random.default_rng(23)
+    nt = 1500
+    t = (np.arange(nt) * 0.0204).astype(np.float32)
+    x = (1.0 + 1e-3 * rng.standard_normal(nt)).astype(np.float32)
+    x[(t > 9.0) & (t < 9.15)] += 3e-3
+    kw = dict(nw=16, nl=10, nsc=27, wmax=0.5, dw=0.03, deltat=np.float32(0.0204))
+    a = og.gtls(x[:, None], t[:, None], **kw)
+    b = og.gtls64(x[:, None], t[:, None], **kw)
+    pos = (a["tlssn"] > 0) & (b["tlssn"] > 0)
+    rel = np.abs(a["tlssn"][pos] - b["sn64"][pos]) / b["sn64"][pos]
+    assert np.median(rel) < 2e-5 and np.mean(a["tlsw"] == b["tlsw"]) > 0.98
