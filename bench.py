@@ -259,6 +259,19 @@ def bench_serve(steps: int, warmup: int):
     served_last = len(sub) / walls[-1]
     log(f"plain: {len(sub)} requests in {pwall:.3f} s (served the same in {walls[-1]:.3f} s); "
         f"results identical: {same}")
+    # detection with the least-squares statistic on the last traces (served
+    # again, untimed)
+    import dataclasses
+    srv_d = TransitServer(dataclasses.replace(cfg, stat="dchi2"))
+    rec_d = ninj_d = 0
+    ndet = min(steps, 8)
+    for k in range(warmup + steps - ndet, warmup + steps):
+        reqs, truth = traces[k]
+        for r in srv_d.run(reqs, realtime=False):
+            if truth[r.rid] is not None:
+                ninj_d += 1
+                rec_d += synth.recovered(r.best_period, truth[r.rid])
+    log(f"detection: tls.py S/N {rec}/{ninj}, least-squares statistic {rec_d}/{ninj_d}")
     # dominant kernel: the serving steps' tlsfold (item-list instantiation),
     # every launch of the run (warm-up included, as a profiler sees them)
     # bracketed by HIP events on the serving stream
@@ -297,7 +310,11 @@ def bench_serve(steps: int, warmup: int):
                              "(80 ms at 2000 req/s) that drain before the next burst; "
                              "open-loop figures: --workload openloop"),
         detection=dict(injected=ninj, recovered=rec, fraction=rec / max(ninj, 1),
-                       rule="best period within 1% of P, P/2, P/3, 2P or 3P"),
+                       rule="best period within 1% of P, P/2, P/3, 2P or 3P",
+                       statistic="tls.py S/N (the served default)",
+                       least_squares=dict(injected=ninj_d, recovered=rec_d,
+                                          fraction=rec_d / max(ninj_d, 1),
+                                          traces=ndet, statistic="stat=dchi2")),
         config=dict(workload="serving ragged TLS+detrend: Poisson 2000 req/s, n log-uniform "
                              "2048-70000 at 0.0204 d, periods geomspace(0.5, baseline/3, "
                              "500-4000), 10 W x 10 L, detrend r=64 divide",
@@ -401,6 +418,8 @@ def bench_tls1(steps: int, warmup: int):
     sn = out["sn"].cpu().numpy()
     from oracle import tlsfold as ot
     best, bval, sde = ot.tlsstats(sn)
+    snd = tlsfold.tlsfold_device(batch, widths, nl, synth.CADENCE, stat="dchi2")["sn"]
+    bd, vd, sded = ot.tlsstats(snd.cpu().numpy())
     trials = int(np.sum(np.ceil(per / synth.CADENCE))) * len(widths) * nl
     evaluated = trials * (nl - 1) / nl
     ach = TRIAL_FLOPS * evaluated / (kms * 1e-3) / 1e12
@@ -412,7 +431,11 @@ def bench_tls1(steps: int, warmup: int):
                 dtype="f64", trial_evals_per_s=trials / (kms * 1e-3),
                 detection=dict(best_period=float(per[best]), injected_period=d["truth"]["P"],
                                recovered=bool(synth.recovered(per[best], d["truth"])),
-                               sn=float(bval), sde=float(sde)),
+                               sn=float(bval), sde=float(sde), statistic="tls.py S/N",
+                               least_squares=dict(best_period=float(per[bd]),
+                                                  recovered=bool(synth.recovered(per[bd],
+                                                                                 d["truth"])),
+                                                  value=float(vd), sde=float(sded))),
                 config=dict(workload="plain TLS: 1 curve x 4096 samples at 0.0204 d, trapezoid "
                                      "depth 2e-3 at P 3.7 d, 1000 periods 0.5-20 d, 10 W x 10 L",
                             n=len(t), periods=len(per)),

@@ -213,11 +213,17 @@ __device__ __forceinline__ void tf_window(double mcnt, double sM, double nAll, d
 // sums, so res denA^2 = y2 denA^2 - numB denA U + numB^2 V, and
 // S/N = -H / sqrt(res/(n-3)) has the rank key
 // k = sign(numB denA) numB^2 (n-3) / (res denA^2): one division per trial.
+// With stat = 1 the trial is the least-squares fit of the same trapezoid
+// shape instead (the detection statistic): res(H) = y2 + H U + H^2 V is
+// minimal at H* = -U/(2V), where the chi^2 improvement is U^2/(4V); the key
+// is sign(-H*) U^2/(4V) = sign(U) U^2/(4V) (no division by the window's own
+// residual: the reported value divides sqrt(|key|) by the curve's standard
+// deviation), and numB and denA are U and 2V (H = -numB/denA = H*).
 // c is the trial's (W, L) table row (products of L and W at 4..9).  Writes
 // numB and denA; NaN for no trial (n <= 3, denA = 0).  Multiply-adds may
 // fuse here: the statistic's rounding is not part of any region membership.
 __device__ __forceinline__ double tf_eval(const TfRegions &g, double W, const double *c,
-                                          double *numBout, double *denAout) {
+                                          int stat, double *numBout, double *denAout) {
 #pragma clang fp contract(fast)
   const double c4L2 = c[4], c8L2 = c[5], c4WL = c[6], c16L = c[7], c2iL = c[8], ciL2 = c[9];
   const double q4 = 0.25 * W, q42 = q4 * q4, W2 = W * W, W8 = 8.0 * W;
@@ -241,6 +247,18 @@ __device__ __forceinline__ double tf_eval(const TfRegions &g, double W, const do
   *numBout = numB;
   *denAout = denA;
   constexpr double kNaN = __builtin_nan("");
+  if (stat) {
+    // least-squares matched filter of the trapezoid shape: signed chi^2
+    // improvement (the division by 4V is the same for every trial's rank
+    // only up to V, so it stays)
+    const double k = U * U / (4.0 * V);
+    double key = U < 0.0 ? -k : k;
+    if (nn <= 3.0 || V == 0.0) key = kNaN;
+    *numBout = U;
+    *denAout = 2.0 * V;
+    return key;
+  }
+  // (S/N)^2 = H^2 (n-3)/res = numB^2 (n-3)/R
   const double s2 = numB * numB * (nn - 3.0);
   const bool neg = (numB < 0.0) != (denA < 0.0);
   const double k = s2 / R;  // every case below is a select (no branch)
@@ -266,7 +284,7 @@ __device__ __forceinline__ double tf_h(double numB, double denA) { return -numB 
 __device__ __forceinline__ double tf_stat(const TfSums &c0, const TfSums &cAB,
                                           const TfSums &cBC, const TfSums &cCD,
                                           const TfSums &cDE, const TfSums &cE,
-                                          const TfSums &cM, double W, const double *c,
+                                          const TfSums &cM, double W, const double *c, int stat,
                                           double *numB, double *denA) {
   TfRegions g;
   tf_window(cM.n - c0.n, cM.s1 - c0.s1, cE.n - c0.n, cE.s1 - c0.s1, cE.s2 - c0.s2, g.mean, g.y2);
@@ -277,7 +295,7 @@ __device__ __forceinline__ double tf_stat(const TfSums &c0, const TfSums &cAB,
   g.dB = cBC.d1 - cAB.d1; g.dD = cDE.d1 - cCD.d1;
   g.d2B = cBC.d2 - cAB.d2; g.d2D = cDE.d2 - cCD.d2;
   g.xdB = cBC.xd - cAB.xd; g.xdD = cDE.xd - cCD.xd;
-  return tf_eval(g, W, c, numB, denA);
+  return tf_eval(g, W, c, stat, numB, denA);
 }
 
 struct TfArgs {
@@ -305,6 +323,7 @@ struct TfArgs {
   const int32_t *nitems_dev;
   int32_t *counter;
   int32_t te_run;  // epochs per window-table run (fits the dynamic LDS)
+  int32_t stat;    // 0: tls.py's S/N; 1: the chi^2-improvement detection statistic
 };
 
 // LDS layout for a staged range
@@ -608,7 +627,7 @@ __device__ __forceinline__ int tf_bisect(const TfLds &L, int a, int b, double t0
 // only for B and D), shifted to epoch coordinates (d = q - s) per region.
 __device__ __forceinline__ double tf_stat_px(const TfLds &L, int p0, int pAB, int pBC, int pCD,
                                              int pDE, int pE, double mean, double y2, double s,
-                                             double W, const double *c, double *numB,
+                                             double W, const double *c, int stat, double *numB,
                                              double *denA) {
 #pragma clang fp contract(fast)
   const double *r0 = tf_px(L, p0), *rAB = tf_px(L, pAB), *rBC = tf_px(L, pBC),
@@ -628,7 +647,7 @@ __device__ __forceinline__ double tf_stat_px(const TfLds &L, int p0, int pAB, in
   g.dD = q1D - g.nD * s;
   g.d2D = q2D - 2.0 * s * q1D + g.nD * s * s;
   g.xdD = xqD - s * g.sD;
-  return tf_eval(g, W, c, numB, denA);
+  return tf_eval(g, W, c, stat, numB, denA);
 }
 
 static_assert(sizeof(TfLds) >= (kTfMaxBuckets + 1) * sizeof(int), "bucket counters exceed LDS");
@@ -643,6 +662,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
   __shared__ TfLds L;
   __shared__ int s_cnt, s_i0, s_wsum[kTfThreads / kWave], s_brk[kTfMaxBreaks], s_nbrk;
   __shared__ int64_t s_item;
+  __shared__ double s_sigma;  // the curve's flux standard deviation (stat 1)
   extern __shared__ __align__(16) unsigned char tf_dyn[];
   const int tid = threadIdx.x;
   const int nw = A.nw, nl = A.nl;
@@ -742,6 +762,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
     if (tid == 0) { s_cnt = 0; s_i0 = n; }
     for (int b = tid; b <= nb; b += kTfThreads) lcur[b] = 0;
     __syncthreads();
+    double mysx = 0.0, mysxx = 0.0;  // flux sums for the curve's deviation (stat 1)
     {
       int myv = 0, myi0 = n;
       for (int i0 = tid; i0 < n; i0 += kTfUnroll * kTfThreads) {
@@ -758,6 +779,10 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
           if (i >= n || !tf_usable(tv[v], xv[v])) continue;
           ++myv;
           myi0 = min(myi0, i);
+          if (A.stat) {
+            mysx += (double)xv[v];
+            mysxx += (double)xv[v] * (double)xv[v];
+          }
           const double u = tf_fmod((double)tv[v], P, invP);
           atomicAdd(&lcur[min(nb - 1, (int)(u * bscale))], 1);
         }
@@ -765,8 +790,27 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
       atomicAdd(&s_cnt, myv);
       atomicMin(&s_i0, myi0);
     }
+    if (A.stat) {
+      // the curve's standard deviation over its usable samples, reduced in a
+      // fixed order (wave shuffles, then the waves in turn): every item of a
+      // curve gets the same bits
+      for (int o = kWave / 2; o > 0; o >>= 1) {
+        mysx += __shfl_xor(mysx, o);
+        mysxx += __shfl_xor(mysxx, o);
+      }
+      if ((tid & (kWave - 1)) == 0) {
+        L.red_key[tid / kWave] = mysx;
+        L.red_nb[tid / kWave] = mysxx;
+      }
+    }
     __syncthreads();
     const int nv = s_cnt;
+    if (A.stat && tid == 0) {  // read by thread 0 only, when it writes the item
+      double sx = 0.0, sxx = 0.0;
+      for (int w = 0; w < kTfThreads / kWave; ++w) { sx += L.red_key[w]; sxx += L.red_nb[w]; }
+      const double cn = (double)nv;
+      s_sigma = nv > 1 ? sqrt(fmax(sxx - sx * sx / cn, 0.0) / (cn - 1.0)) : NAN;
+    }
     // x_ref = flux of the first usable sample (any constant cancels in the statistic)
     const double xref = (s_i0 < n) ? (double)xc[s_i0] : 0.0;
     {  // exclusive scan of lcur[0..nb) in place (per-thread runs + wave scan)
@@ -979,7 +1023,8 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
             const int pDE = tf_clamp(br[3], pCD, pE);
             double nB, dA;
             const double key =
-                tf_stat_px(L, p0, pAB, pBC, pCD, pDE, pE, wd[0], wd[1], s, W, bd, &nB, &dA);
+                tf_stat_px(L, p0, pAB, pBC, pCD, pDE, pE, wd[0], wd[1], s, W, bd, A.stat, &nB,
+                           &dA);
             const unsigned long long idx = tf_idx(w, m, ep, nl);
             if (tf_better(key, idx, best)) best = TfBest{key, nB, dA, idx};
             // next trial (wrap by an unsigned min: m - n wraps above m when m < n)
@@ -1064,7 +1109,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
           const double key =
               W < P ? tf_stat(get(0), get(3 + 4 * mm), get(4 + 4 * mm), get(5 + 4 * mm),
                               get(6 + 4 * mm), get(1), get(2), W, wlt + kTfWl * (w * nl + m),
-                              &nB, &dA)
+                              A.stat, &nB, &dA)
                     : __builtin_nan("");  // W >= P: no trial
           const unsigned long long idx = tf_idx(w, m, e0 + el, nl);
           if (tf_better(key, idx, best)) best = TfBest{key, nB, dA, idx};
@@ -1097,7 +1142,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
           b = TfBest{L.red_key[k], L.red_nb[k], L.red_da[k], L.red_idx[k]};
       const bool found = b.idx != 0ull;
       const int wl = found ? (int)((b.idx - 1ull) >> 32) : -1;
-      A.sn[item] = tf_sn(b.key);
+      A.sn[item] = A.stat ? tf_sn(b.key) / s_sigma : tf_sn(b.key);
       if (A.hbest) A.hbest[item] = found ? tf_h(b.numB, b.denA) : NAN;
       A.bw[item] = found ? wl / nl : -1;
       A.bl[item] = found ? wl % nl : -1;
@@ -1156,7 +1201,8 @@ extern "C" int gtrap_tlsfold(double *snr, double *hbest, int32_t *best_w, int32_
                              const int32_t *len, int32_t nmax, const double *periods,
                              const int64_t *poff, int32_t ncurves, int64_t nptot,
                              const double *widths, int32_t nw, int32_t nl, double dtep,
-                             void *ws, int64_t ws_bytes, int32_t nblocks, void *stream) {
+                             int32_t stat, void *ws, int64_t ws_bytes, int32_t nblocks,
+                             void *stream) {
   GTRAP_REQUIRE(snr && best_w && best_l && best_e && t && x && off && len && periods && poff &&
                     widths && ws,
                 "gtrap_tlsfold: null pointer");
@@ -1166,6 +1212,8 @@ extern "C" int gtrap_tlsfold(double *snr, double *hbest, int32_t *best_w, int32_
   GTRAP_REQUIRE(nl >= 1 && nl <= gtrap::kTfMaxNl, "gtrap_tlsfold: nl must be 1..%d",
                 gtrap::kTfMaxNl);
   GTRAP_REQUIRE(dtep > 0.0, "gtrap_tlsfold: dtep must be positive");
+  GTRAP_REQUIRE(stat == GTRAP_TLS_SNR || stat == GTRAP_TLS_DCHI2,
+                "gtrap_tlsfold: stat must be GTRAP_TLS_SNR or GTRAP_TLS_DCHI2");
   GTRAP_REQUIRE(nmax >= 1, "gtrap_tlsfold: nmax must be >= 1");
   GTRAP_REQUIRE(nblocks >= 1, "gtrap_tlsfold: nblocks must be >= 1");
   GTRAP_REQUIRE(ws_bytes >= gtrap_tlsfold_ws_bytes(nmax, nblocks),
@@ -1176,7 +1224,7 @@ extern "C" int gtrap_tlsfold(double *snr, double *hbest, int32_t *best_w, int32_
   if (const int rc = gtrap::tf_launch_cfg<false>(nw, nl, &dyn, &te_run)) return rc;
   gtrap::TfArgs A{t, x, off, len, periods, poff, nullptr, ncurves, nptot, widths, nw, nl, dtep,
                   snr, hbest, best_w, best_l, best_e, (unsigned char *)ws,
-                  tf_slot_bytes(nmax), nmax, nullptr, nullptr, nullptr, nullptr, te_run};
+                  tf_slot_bytes(nmax), nmax, nullptr, nullptr, nullptr, nullptr, te_run, stat};
   const int64_t g = nptot < nblocks ? nptot : nblocks;
   hipStream_t st = gtrap::as_stream(stream);
   hipLaunchKernelGGL(gtrap::tf_zero_ticket, dim3(1), dim3(1), 0, st, (int *)ws);
@@ -1233,8 +1281,8 @@ extern "C" int gtrap_tlsfold_static(double *snr, double *hbest, int32_t *best_w,
                                     const float *x, const int64_t *off, const int32_t *len,
                                     int32_t nmax, const double *periods, const int64_t *poff,
                                     int32_t ncurves, int64_t nptot, const double *widths,
-                                    int32_t nw, int32_t nl, double dtep, int32_t nblocks,
-                                    void *stream) {
+                                    int32_t nw, int32_t nl, double dtep, int32_t stat,
+                                    int32_t nblocks, void *stream) {
   GTRAP_REQUIRE(nmax >= 1 && nblocks >= 1, "gtrap_tlsfold_static: bad nmax or nblocks");
   const int64_t slot = tf_slot_bytes(nmax);
   GTRAP_REQUIRE(256 + slot <= gtrap::kTfStaticBytes,
@@ -1246,8 +1294,8 @@ extern "C" int gtrap_tlsfold_static(double *snr, double *hbest, int32_t *best_w,
   const int64_t fit = (gtrap::kTfStaticBytes - 256) / slot;
   const int32_t nb = (int64_t)nblocks < fit ? nblocks : (int32_t)fit;
   return gtrap_tlsfold(snr, hbest, best_w, best_l, best_e, t, x, off, len, nmax, periods, poff,
-                       ncurves, nptot, widths, nw, nl, dtep, ws, gtrap::kTfStaticBytes, nb,
-                       stream);
+                       ncurves, nptot, widths, nw, nl, dtep, stat, ws, gtrap::kTfStaticBytes,
+                       nb, stream);
 }
 
 extern "C" int gtrap_tlsfold_items(double *snr, double *hbest, int32_t *best_w, int32_t *best_l,
@@ -1256,8 +1304,8 @@ extern "C" int gtrap_tlsfold_items(double *snr, double *hbest, int32_t *best_w, 
                                    const double *periods, const int32_t *item_curve,
                                    const int64_t *item_pidx, const int32_t *nitems_dev,
                                    int32_t *counter, const double *widths, int32_t nw,
-                                   int32_t nl, double dtep, void *ws, int64_t ws_bytes,
-                                   int32_t nblocks, void *stream) {
+                                   int32_t nl, double dtep, int32_t stat, void *ws,
+                                   int64_t ws_bytes, int32_t nblocks, void *stream) {
   GTRAP_REQUIRE(snr && best_w && best_l && best_e && t && x && off && len && periods &&
                     item_curve && item_pidx && nitems_dev && counter && widths && ws,
                 "gtrap_tlsfold_items: null pointer");
@@ -1266,6 +1314,8 @@ extern "C" int gtrap_tlsfold_items(double *snr, double *hbest, int32_t *best_w, 
   GTRAP_REQUIRE(nl >= 1 && nl <= gtrap::kTfMaxNl, "gtrap_tlsfold_items: nl must be 1..%d",
                 gtrap::kTfMaxNl);
   GTRAP_REQUIRE(dtep > 0.0 && nmax >= 1 && nblocks >= 1, "gtrap_tlsfold_items: bad scalars");
+  GTRAP_REQUIRE(stat == GTRAP_TLS_SNR || stat == GTRAP_TLS_DCHI2,
+                "gtrap_tlsfold_items: stat must be GTRAP_TLS_SNR or GTRAP_TLS_DCHI2");
   GTRAP_REQUIRE(ws_bytes >= gtrap_tlsfold_ws_bytes(nmax, nblocks),
                 "gtrap_tlsfold_items: workspace too small");
   size_t dyn = 0;
@@ -1273,7 +1323,8 @@ extern "C" int gtrap_tlsfold_items(double *snr, double *hbest, int32_t *best_w, 
   if (const int rc = gtrap::tf_launch_cfg<true>(nw, nl, &dyn, &te_run)) return rc;
   gtrap::TfArgs A{t, x, off, len, periods, nullptr, nullptr, 0, 0, widths, nw, nl, dtep,
                   snr, hbest, best_w, best_l, best_e, (unsigned char *)ws,
-                  tf_slot_bytes(nmax), nmax, item_curve, item_pidx, nitems_dev, counter, te_run};
+                  tf_slot_bytes(nmax), nmax, item_curve, item_pidx, nitems_dev, counter, te_run,
+                  stat};
   hipLaunchKernelGGL(gtrap::tlsfold_kernel<true>, dim3((unsigned)nblocks), dim3(gtrap::kTfThreads),
                      dyn, gtrap::as_stream(stream), A);
   return gtrap::check_launch("tlsfold_kernel<items>");

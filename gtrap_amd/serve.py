@@ -50,7 +50,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _native
-from .tlsfold import Workspace
+from .tlsfold import STATS, Workspace
 
 SAMPLE_BYTES = 12   # t, flux, detrended flux (float32 each)
 PERIOD_BYTES = 36   # period f64, S/N f64, H f64, w/l/e int32
@@ -164,7 +164,8 @@ class PlainSearch:
 
 
 def search_plain(req: Request, widths, nl=10, dtep=0.0204, r=64, mode="div", nblocks=512,
-                 ctx: PlainSearch | None = None, periodogram: bool = False) -> Result:
+                 ctx: PlainSearch | None = None, periodogram: bool = False,
+                 stat: str = "snr") -> Result:
     import torch
     from . import gfilter
     from .tlsfold import FoldBatch, tlsfold_device
@@ -181,7 +182,7 @@ def search_plain(req: Request, widths, nl=10, dtep=0.0204, r=64, mode="div", nbl
     ln = torch.full((1,), n, dtype=torch.int32, device="cuda")
     poff = torch.tensor([0, m], dtype=torch.int64, device="cuda")
     batch = FoldBatch(t, xd, off, ln, per, poff, 1, m, n, np.array([0, m]))
-    out = tlsfold_device(batch, widths, nl, dtep, nblocks=nblocks, workspace=ctx.ws)
+    out = tlsfold_device(batch, widths, nl, dtep, nblocks=nblocks, workspace=ctx.ws, stat=stat)
     bp = torch.empty(1, dtype=torch.int64, device="cuda")
     bs = torch.empty(1, dtype=torch.float64, device="cuda")
     sde = torch.empty(1, dtype=torch.float64, device="cuda")
@@ -229,6 +230,7 @@ class ServerConfig:
     seg_len: int = 512               # detrend rows per lane
     use_graph: bool = True
     periodogram: bool = False        # return every request's per-period results
+    stat: str = "snr"                # ranking statistic: "snr" (tls.py) or "dchi2"
 
     @classmethod
     def tuned(cls, **kw) -> "ServerConfig":
@@ -400,8 +402,8 @@ class HipBackend:
                      p(self.be), p(self.t), p(self.xd), p(self.d_off), p(self.d_len),
                      cfg.max_len, p(self.per), p(self.d_item_curve), p(self.d_item_pidx),
                      p(cnt[1:2]), p(cnt[3:4]), p(self.widths), int(self.widths.numel()), cfg.nl,
-                     float(cfg.dtep), p(self.ws_buf), int(self.ws_bytes), cfg.nblocks,
-                     _native.stream_handle())
+                     float(cfg.dtep), STATS[cfg.stat], p(self.ws_buf), int(self.ws_bytes),
+                     cfg.nblocks, _native.stream_handle())
 
     def _rec_post(self):
         """Periodogram statistics of the curves finished in this step."""

@@ -86,10 +86,15 @@ def _workspace_for(stream) -> Workspace:
     return _WS[key]
 
 
+STATS = {"snr": 0, "dchi2": 1}
+
+
 def tlsfold_device(batch: FoldBatch, widths, nl: int, dtep: float, out=None,
                    nblocks: int = DEFAULT_BLOCKS, workspace: Workspace | None = None,
-                   stream=None):
-    """Launch on a packed batch; returns device tensors (sn, h, w, l, e)."""
+                   stream=None, stat: str = "snr"):
+    """Launch on a packed batch; returns device tensors (sn, h, w, l, e).
+    ``stat``: "snr" (tls.py's S/N) or "dchi2" (the detection statistic, the
+    fit's chi^2 improvement over the residual variance; reported in "sn")."""
     import torch
     _native.require_gpu()
     ws = workspace or _workspace_for(stream)
@@ -107,17 +112,17 @@ def tlsfold_device(batch: FoldBatch, widths, nl: int, dtep: float, out=None,
     _native.call("gtrap_tlsfold", p(out["sn"]), p(out["h"]), p(out["w"]), p(out["l"]),
                  p(out["e"]), p(batch.t), p(batch.x), p(batch.off), p(batch.length),
                  int(batch.nmax), p(batch.periods), p(batch.poff), int(batch.ncurves),
-                 int(batch.nptot), p(wdev), int(wdev.numel()), int(nl), float(dtep), p(buf),
-                 int(need), nb, _native.stream_handle(stream))
+                 int(batch.nptot), p(wdev), int(wdev.numel()), int(nl), float(dtep),
+                 STATS[stat], p(buf), int(need), nb, _native.stream_handle(stream))
     return out
 
 
 def tlsfold(curves, periods_list, widths, nl: int = 10, dtep: float = 0.0204,
-            nblocks: int = DEFAULT_BLOCKS):
+            nblocks: int = DEFAULT_BLOCKS, stat: str = "snr"):
     """Host convenience: returns a list of dicts (sn, h, w, l, e) per curve."""
     import torch
     batch = pack(curves, periods_list)
-    out = tlsfold_device(batch, widths, nl, dtep, nblocks=nblocks)
+    out = tlsfold_device(batch, widths, nl, dtep, nblocks=nblocks, stat=stat)
     torch.cuda.synchronize()
     host = {k: v.cpu().numpy() for k, v in out.items()}
     res = []

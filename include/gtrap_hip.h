@@ -293,6 +293,8 @@ int gtrap_tlsgrid(const double *t, const double *x, int32_t n, const double *Wl,
                   double *xsum, void *stream);
 
 /* ---- periodic trapezoid least squares (tls_fold) --------------------------- */
+#define GTRAP_TLS_SNR   0  /* tls.py's S/N = -H / sqrt(res/(n-3)) (tls.py:100) */
+#define GTRAP_TLS_DCHI2 1  /* least-squares detection statistic, see gtrap_tlsfold */
 
 /*
  * Periodic extension of the trapezoid statistic of gtrap/tls.py:44-110
@@ -309,6 +311,17 @@ int gtrap_tlsgrid(const double *t, const double *x, int32_t n, const double *Wl,
  * arrays; its periods are periods[poff[c] .. poff[c+1]) with poff[ncurves] =
  * nptot.  Outputs per period (index poff[c] + p): best S/N, its H, and the
  * w, m, e indices (-1 when every trial is NaN).
+ * stat selects the ranking statistic, reported in snr: GTRAP_TLS_SNR (the
+ * reference's S/N, default) or GTRAP_TLS_DCHI2, no counterpart in the
+ * reference: the least-squares fit of the same trapezoid shape g (tls.py:2-15
+ * with unit depth) to y = x - mean over the window [-W/2, W/2),
+ * H* = sum(y g)/sum(g^2), D = sign(-H*) sqrt(dchi2 (n-3)/res*) with
+ * dchi2 = sum(y g)^2/sum(g^2) and res* = sum(y^2) - dchi2 (a matched
+ * filter); hbest then holds H*.  tls.py's S/N is H over the per-sample
+ * residual RMS, so it does not grow with the number of transits folded into
+ * a window; D does (about depth/sigma times the square root of the
+ * in-transit samples), which makes it the statistic that recovers injected
+ * periods.  L = 0 (m = 0) is no trial in both.
  * Precondition len[c] <= nmax: each workgroup's workspace slot holds nmax
  * samples.  The lengths live on the device, so the kernel checks it: the
  * periods of a longer curve are not searched and get S/N NaN, H NaN and
@@ -321,7 +334,7 @@ int gtrap_tlsfold(double *snr, double *hbest, int32_t *best_w, int32_t *best_l,
                   int32_t *best_e, const float *t, const float *x, const int64_t *off,
                   const int32_t *len, int32_t nmax, const double *periods,
                   const int64_t *poff, int32_t ncurves, int64_t nptot,
-                  const double *widths, int32_t nw, int32_t nl, double dtep,
+                  const double *widths, int32_t nw, int32_t nl, double dtep, int32_t stat,
                   void *ws, int64_t ws_bytes, int32_t nblocks, void *stream);
 
 /*
@@ -344,7 +357,7 @@ int gtrap_tlsfold_static(double *snr, double *hbest, int32_t *best_w, int32_t *b
                          const int32_t *len, int32_t nmax, const double *periods,
                          const int64_t *poff, int32_t ncurves, int64_t nptot,
                          const double *widths, int32_t nw, int32_t nl, double dtep,
-                         int32_t nblocks, void *stream);
+                         int32_t stat, int32_t nblocks, void *stream);
 
 /*
  * Item-list form of gtrap_tlsfold for the serving loop (no reference
@@ -360,8 +373,8 @@ int gtrap_tlsfold_items(double *snr, double *hbest, int32_t *best_w, int32_t *be
                         const int32_t *len, int32_t nmax, const double *periods,
                         const int32_t *item_curve, const int64_t *item_pidx,
                         const int32_t *nitems_dev, int32_t *counter, const double *widths,
-                        int32_t nw, int32_t nl, double dtep, void *ws, int64_t ws_bytes,
-                        int32_t nblocks, void *stream);
+                        int32_t nw, int32_t nl, double dtep, int32_t stat, void *ws,
+                        int64_t ws_bytes, int32_t nblocks, void *stream);
 
 /*
  * Periodogram statistics of the tls_fold output, the TLS counterpart of

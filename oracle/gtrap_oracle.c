@@ -174,6 +174,42 @@ void orc_geebls(float *blsp, uint32_t *phase1, uint32_t *phase2, const float *lc
 /* degrees of freedom) is NaN, and so is one with W >= P (a window as long  */
 /* as the period would hold two images of a sample).                        */
 /* ------------------------------------------------------------------------ */
+/* The value of a trial.  stat 0: tls.py's S/N = -H/sqrt(res/(n-3))
+ * (tls.py:100).  stat 1: the least-squares fit of the trapezoid shape g
+ * (tls.py:2-15 with unit depth), from a = sum g^2 and b = sum y g over the
+ * window: H* = b/a and the signed chi^2 improvement sign(-H*) b^2/a, which
+ * ranks the trials (the caller divides its root by the curve's standard
+ * deviation to report it). */
+static double trial_value(double H, double res, double a, double b, int n, int stat,
+                          double *Hls) {
+  if (!stat) return -H / sqrt(res / (double)(n - 3));
+  if (!(a > 0.0)) return NAN;
+  *Hls = b / a;
+  const double k = b * b / a;
+  return b > 0.0 ? -k : k;
+}
+
+/* root of a signed chi^2 improvement over sigma (stat 1's reported value) */
+static double dchi2_value(double key, double sigma) {
+  return (key >= 0.0 ? sqrt(key) : -sqrt(-key)) / sigma;
+}
+
+/* standard deviation of the usable samples' flux (sum of squares form, the
+ * kernel's) */
+static double flux_sigma(const float *t, const float *x, int n) {
+  double sx = 0.0, sxx = 0.0;
+  int c = 0;
+  for (int i = 0; i < n; ++i) {
+    if (!(t[i] >= 0.0f && isfinite(x[i]))) continue;
+    sx += (double)x[i];
+    sxx += (double)x[i] * (double)x[i];
+    c++;
+  }
+  if (c <= 1) return NAN;
+  const double v = sxx - sx * sx / (double)c;
+  return sqrt((v > 0.0 ? v : 0.0) / (double)(c - 1));
+}
+
 /* a sample takes part when its time is valid (t >= 0) and its flux finite */
 static int usable(float t, float x) { return t >= 0.0f && isfinite(x); }
 
@@ -185,8 +221,8 @@ static double fold_offset(double u, double t0, double P) {
 }
 
 double orc_tlsfold_trial(const float *t, const float *x, int n, double P, int e, double W,
-                         int m, int nl, double dtep, int divx, double *Hout, int *cnt,
-                         double *resout, double *meanout) {
+                         int m, int nl, double dtep, int divx, int stat, double *Hout,
+                         int *cnt, double *resout, double *meanout) {
   const double t0 = (double)e * dtep;
   const double dl = 0.45 * W / (double)nl;
   const double L = (double)m * dl;
@@ -217,19 +253,21 @@ double orc_tlsfold_trial(const float *t, const float *x, int n, double P, int e,
   const double numB = 8.0 * L * L * xAEC - 4.0 * W * L * xBD + 16.0 * L * (xtD - xtB);
   const double H = -numB / denA;
   const double h = H / 2.0;
-  double res = 0.0;
+  double res = 0.0, ga = 0.0, gb = 0.0;
   for (int i = 0; i < n; ++i) {
     if (!usable(t[i], x[i])) continue;
     const double d = fold_offset(fmod((double)t[i], P), t0, P);
     const double y = (double)x[i] - mean;
-    double r;
-    if (d >= -hw && d < tAB) r = y + h;
-    else if (d >= tAB && d < tBC) r = y - H / L * (d + W / 4.0);
-    else if (d >= tBC && d < tCD) r = y - h;
-    else if (d >= tCD && d < tDE) r = y + H / L * (d - W / 4.0);
-    else if (d >= tDE && d < hw) r = y + h;
+    double r, gv;  /* gv: the trapezoid of unit depth, tls.py:2-15 */
+    if (d >= -hw && d < tAB) { r = y + h; gv = -0.5; }
+    else if (d >= tAB && d < tBC) { r = y - H / L * (d + W / 4.0); gv = (d + W / 4.0) / L; }
+    else if (d >= tBC && d < tCD) { r = y - h; gv = 0.5; }
+    else if (d >= tCD && d < tDE) { r = y + H / L * (d - W / 4.0); gv = -(d - W / 4.0) / L; }
+    else if (d >= tDE && d < hw) { r = y + h; gv = -0.5; }
     else continue;
     res += r * r;
+    ga += gv * gv;
+    gb += y * gv;
   }
   const int nn = nA + nB + nC + nD + nE;
   if (Hout) *Hout = H;
@@ -238,13 +276,17 @@ double orc_tlsfold_trial(const float *t, const float *x, int n, double P, int e,
   if (meanout) *meanout = mean;
   if (nn <= 3) return NAN; /* n - 3 degrees of freedom: no trial below 4 samples */
   if (!(W < P)) return NAN; /* no trial: the window spans the period */
-  return -H / sqrt(res / (double)(nn - 3));
+  if (m == 0) return NAN;   /* L = 0: no trial (tls.py's form gives 0/0) */
+  double Hls = H;
+  const double v = trial_value(H, res, ga, gb, nn, stat, &Hls);
+  if (Hout) *Hout = Hls;
+  return stat ? dchi2_value(v, flux_sigma(t, x, n)) : v;
 }
 
 /* Window-restricted version of the trial above: the same sums over the
  * samples whose offsets were precomputed (d[k], y-values via x[k]). */
 static double trial_window(const double *d, const double *xv, int k, double W, int m, int nl,
-                           double *Hout) {
+                           int stat, double *Hout) {
   const double dl = 0.45 * W / (double)nl;
   const double L = (double)m * dl;
   const double hw = 0.5 * W, qw = 0.25 * W, hl = 0.5 * L;
@@ -269,22 +311,27 @@ static double trial_window(const double *d, const double *xv, int k, double W, i
   const double numB = 8.0 * L * L * xAEC - 4.0 * W * L * xBD + 16.0 * L * (xtD - xtB);
   const double H = -numB / denA;
   const double h = H / 2.0;
-  double res = 0.0;
+  double res = 0.0, ga = 0.0, gb = 0.0;
   for (int i = 0; i < k; ++i) {
     const double dd = d[i], y = xv[i] - mean;
-    double r;
-    if (dd >= -hw && dd < tAB) r = y + h;
-    else if (dd >= tAB && dd < tBC) r = y - H / L * (dd + W / 4.0);
-    else if (dd >= tBC && dd < tCD) r = y - h;
-    else if (dd >= tCD && dd < tDE) r = y + H / L * (dd - W / 4.0);
-    else if (dd >= tDE && dd < hw) r = y + h;
+    double r, gv;  /* gv: the trapezoid of unit depth, tls.py:2-15 */
+    if (dd >= -hw && dd < tAB) { r = y + h; gv = -0.5; }
+    else if (dd >= tAB && dd < tBC) { r = y - H / L * (dd + W / 4.0); gv = (dd + W / 4.0) / L; }
+    else if (dd >= tBC && dd < tCD) { r = y - h; gv = 0.5; }
+    else if (dd >= tCD && dd < tDE) { r = y + H / L * (dd - W / 4.0); gv = -(dd - W / 4.0) / L; }
+    else if (dd >= tDE && dd < hw) { r = y + h; gv = -0.5; }
     else continue;
     res += r * r;
+    ga += gv * gv;
+    gb += y * gv;
   }
   const int nn = nA + nB + nC + nD + nE;
   if (Hout) *Hout = H;
-  if (nn <= 3) return NAN;
-  return -H / sqrt(res / (double)(nn - 3));
+  if (nn <= 3 || m == 0) return NAN;
+  double Hls = H;
+  const double v = trial_value(H, res, ga, gb, nn, stat, &Hls);
+  if (Hout) *Hout = Hls;
+  return v;
 }
 
 typedef struct { double u; double x; } ux_t;
@@ -303,11 +350,12 @@ static int lower_u(const ux_t *s, int k, double v) { /* first index with u >= v 
 }
 
 void orc_tlsfold(const float *t, const float *x, int n, const double *periods, int np,
-                 const double *widths, int nw, int nl, double dtep, double *best_sn,
+                 const double *widths, int nw, int nl, double dtep, int stat, double *best_sn,
                  int *best_w, int *best_l, int *best_e, double *best_h) {
   double wmax = 0.0;
   for (int w = 0; w < nw; ++w) if (widths[w] > wmax) wmax = widths[w];
   const double hwm = 0.5 * wmax;
+  const double sig = stat ? flux_sigma(t, x, n) : 1.0;
   ux_t *s = (ux_t *)malloc(sizeof(ux_t) * (size_t)(n > 0 ? n : 1));
   double *d = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
   double *xv = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
@@ -349,14 +397,14 @@ void orc_tlsfold(const float *t, const float *x, int n, const double *periods, i
       for (int w = 0; w < nw; ++w) {
         for (int m = 0; m < nl; ++m) {
           double H;
-          const double sn = widths[w] < P ? trial_window(d, xv, kk, widths[w], m, nl, &H)
+          const double sn = widths[w] < P ? trial_window(d, xv, kk, widths[w], m, nl, stat, &H)
                                           : (H = NAN, NAN);
           const int earlier = (w < bw) || (w == bw && (m < bl || (m == bl && e < be)));
           if (sn > bsn || (sn == bsn && earlier)) { bsn = sn; bw = w; bl = m; be = e; bh = H; }
         }
       }
     }
-    best_sn[p] = bsn;
+    best_sn[p] = (stat && bw >= 0) ? dchi2_value(bsn, sig) : bsn;
     best_w[p] = bw;
     best_l[p] = bl;
     best_e[p] = be;

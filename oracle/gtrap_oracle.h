@@ -22,13 +22,13 @@ void orc_geebls(float *blsp, uint32_t *phase1, uint32_t *phase2, const float *lc
  * best over (epoch, W, L) per period as tlsfit's nanargmax (tls.py:82-110).
  * Direct evaluation (every trial sums over its window samples). */
 void orc_tlsfold(const float *t, const float *x, int n, const double *periods, int np,
-                 const double *widths, int nw, int nl, double dtep, double *best_sn,
+                 const double *widths, int nw, int nl, double dtep, int stat, double *best_sn,
                  int *best_w, int *best_l, int *best_e, double *best_h);
 /* One trial (period P, epoch e, width W, L index m); returns S/N, writes H,
  * n, the residual res and the window mean (each output pointer may be null). */
 double orc_tlsfold_trial(const float *t, const float *x, int n, double P, int e, double W,
-                         int m, int nl, double dtep, int divx, double *H, int *cnt,
-                         double *res, double *mean);
+                         int m, int nl, double dtep, int divx, int stat, double *H,
+                         int *cnt, double *res, double *mean);
 
 /* include/gtls_simple.h (mode 0) and include/gtls_absolute.h (mode 1), float32
  * line-faithful; outputs at k*nq + q for epochs k in [k0, k1), all curves. */

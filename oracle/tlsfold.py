@@ -18,6 +18,14 @@ a period is the first maximum in (W, L, epoch) order, NaN skipped
 * :func:`trial_res` -- as :func:`trial`, also the residual and window mean.
 * :func:`trial_numpy` -- the tls.py masks evaluated with numpy on folded d.
 * :func:`tlsstats` -- periodogram statistics of one curve (best period, SDE).
+
+``stat="dchi2"`` ranks and reports the detection statistic instead of S/N
+(no counterpart in the reference): the least-squares fit of the trapezoid
+shape g of tls.py:2-15 (unit depth) to y = x - mean over the window,
+H* = sum(y g)/sum(g^2), ranked by the signed chi^2 improvement
+sign(-H*) sum(y g)^2/sum(g^2) and reported as D = sign(-H*) sqrt(improvement)
+/ sigma with sigma the standard deviation of the curve's usable fluxes (a
+matched-filter signal-to-noise ratio); the reported H is H*.
 """
 from __future__ import annotations
 
@@ -28,7 +36,10 @@ import numpy as np
 from ._clib import fptr, lib
 
 
-def tlsfold(t, x, periods, widths, nl, dtep):
+STATS = {"snr": 0, "dchi2": 1}
+
+
+def tlsfold(t, x, periods, widths, nl, dtep, stat="snr"):
     t = np.ascontiguousarray(t, np.float32)
     x = np.ascontiguousarray(x, np.float32)
     periods = np.ascontiguousarray(periods, np.float64)
@@ -40,37 +51,37 @@ def tlsfold(t, x, periods, widths, nl, dtep):
     f = lib().orc_tlsfold
     f.restype = None
     f.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
-                  C.c_int, C.c_double] + [C.c_void_p] * 5
+                  C.c_int, C.c_double, C.c_int] + [C.c_void_p] * 5
     f(fptr(t), fptr(x), len(t), fptr(periods), np_, fptr(widths), len(widths), int(nl),
-      float(dtep), fptr(sn), fptr(bw), fptr(bl), fptr(be), fptr(h))
+      float(dtep), STATS[stat], fptr(sn), fptr(bw), fptr(bl), fptr(be), fptr(h))
     return dict(sn=sn, w=bw, l=bl, e=be, h=h)
 
 
-def _trial(t, x, P, e, W, m, nl, dtep, divx):
+def _trial(t, x, P, e, W, m, nl, dtep, divx, stat="snr"):
     t = np.ascontiguousarray(t, np.float32)
     x = np.ascontiguousarray(x, np.float32)
     f = lib().orc_tlsfold_trial
     f.restype = C.c_double
     f.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int,
-                  C.c_int, C.c_double, C.c_int] + [C.c_void_p] * 4
+                  C.c_int, C.c_double, C.c_int, C.c_int] + [C.c_void_p] * 4
     H = np.zeros(1)
     cnt = np.zeros(1, np.int32)
     res = np.zeros(1)
     mean = np.zeros(1)
     sn = f(fptr(t), fptr(x), len(t), float(P), int(e), float(W), int(m), int(nl), float(dtep),
-           1 if divx else 0, fptr(H), fptr(cnt), fptr(res), fptr(mean))
+           1 if divx else 0, STATS[stat], fptr(H), fptr(cnt), fptr(res), fptr(mean))
     return sn, H[0], int(cnt[0]), res[0], mean[0]
 
 
-def trial(t, x, P, e, W, m, nl, dtep, divx=True):
-    """One trial: (S/N, H, n)."""
-    return _trial(t, x, P, e, W, m, nl, dtep, divx)[:3]
+def trial(t, x, P, e, W, m, nl, dtep, divx=True, stat="snr"):
+    """One trial: (value of the statistic, H, n)."""
+    return _trial(t, x, P, e, W, m, nl, dtep, divx, stat)[:3]
 
 
-def trial_res(t, x, P, e, W, m, nl, dtep, divx=True):
+def trial_res(t, x, P, e, W, m, nl, dtep, divx=True, stat="snr"):
     """One trial: (S/N, H, n, res, mean) with res the residual sum of squares
     of tls.py:74-77 and mean the window mean of tls.py:57-59 (0 without divx)."""
-    return _trial(t, x, P, e, W, m, nl, dtep, divx)
+    return _trial(t, x, P, e, W, m, nl, dtep, divx, stat)
 
 
 def fold_offsets(t, P, e, dtep, x=None):

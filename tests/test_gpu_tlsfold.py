@@ -252,3 +252,23 @@ def test_curve_longer_than_the_workspace_slot_is_refused(gpu):
     assert np.all(np.isnan(sn[2:])) and np.all(w[2:] == -2)
     want = ot.tlsfold(*curves[0], per[0], np.linspace(0.04, 0.3, 10), 10, 0.0204)
     np.testing.assert_allclose(sn[:2], want["sn"], rtol=1e-7)
+
+
+def test_detection_statistic_matches_oracle(gpu):
+    """stat="dchi2" (least-squares matched filter): S/N-like value, H*, and
+    the best (W, L, epoch) against the oracle, tile and chunked paths."""
+    from gtrap_amd import tlsfold
+    widths = np.linspace(0.04, 0.3, 10)
+    curves = [_curve(1500, 81, inject=(1.7, 0.4, 0.12, 0.01, 3e-3)), _curve(12000, 82)]
+    plist = [np.geomspace(0.6, 9.0, 21), np.array([0.5, 0.73, 3.3, 60.0])]
+    got = tlsfold.tlsfold(curves, plist, widths, 10, 0.0204, nblocks=64, stat="dchi2")
+    for (t, x), per, g in zip(curves, plist, got):
+        want = ot.tlsfold(t, x, per, widths, 10, 0.0204, stat="dchi2")
+        fin = np.isfinite(want["sn"])
+        np.testing.assert_allclose(g["sn"][fin], want["sn"][fin], rtol=1e-7)
+        np.testing.assert_allclose(g["h"][fin], want["h"][fin], rtol=1e-7)
+        same = (g["w"] == want["w"]) & (g["l"] == want["l"]) & (g["e"] == want["e"])
+        for p in np.nonzero(~same)[0]:
+            s = ot.trial(t, x, per[p], g["e"][p], widths[g["w"][p]], g["l"][p], 10, 0.0204,
+                         stat="dchi2")[0]
+            assert abs(s - want["sn"][p]) <= 1e-9 * abs(want["sn"][p])

@@ -184,3 +184,60 @@ def test_tlsstats_equals_getstat_formula_on_finite_series():
     pest, sde_b, _, _, idx = og.blsstat(v, None, None, 2048, 1, 1.0, 1.0)
     assert best == int(idx[0])
     np.testing.assert_allclose(sde, sde_b[0], rtol=1e-12)
+
+
+def test_detection_statistic_closed_form():
+    """stat="dchi2": the least-squares fit of the trapezoid shape
+    g = gen_trapzoid(d, 1, W, L, 0) to y = x - mean over the window, computed
+    here directly with numpy: H* = b/a, D = sign(-H*) sqrt(b^2/a) / sigma with
+    sigma the standard deviation of the curve's usable fluxes."""
+    rng = np.random.default_rng(15)
+    n = 900
+    t = np.sort(rng.uniform(0, 30, n)).astype(np.float32)
+    x = (1.0 + 1e-3 * rng.standard_normal(n)).astype(np.float32)
+    t[rng.random(n) < 0.03] = -1000.0
+    sigma = float(np.std(x[t >= 0].astype(np.float64), ddof=1))
+    dtep = 0.0204
+    checked = 0
+    for _ in range(200):
+        P = float(rng.uniform(0.6, 12.0))
+        e = int(rng.integers(0, int(np.ceil(P / dtep))))
+        W = float(rng.uniform(0.04, 0.5))
+        nl, m = 10, int(rng.integers(1, 10))
+        L = m * (0.45 * W / nl)
+        D, Hs, cnt = ot.trial(t, x, P, e, W, m, nl, dtep, stat="dchi2")
+        d, ok = ot.fold_offsets(t, P, e, dtep, x)
+        xx = x[ok].astype(np.float64)
+        mean = xx[(d >= -W / 2) & (d <= W / 2)].mean()
+        win = (d >= -W / 2) & (d < W / 2)
+        y = xx[win] - mean
+        g = gen_trapzoid(d[win], 1.0, W, L, 0.0)
+        a, b = float(g @ g), float(y @ g)
+        if win.sum() <= 3:
+            assert np.isnan(D)
+            continue
+        want = -np.sign(b) * np.sqrt(b * b / a) / sigma
+        # the oracle's sigma is the sum-of-squares form (flux near 1: ~1e-9)
+        np.testing.assert_allclose(D, want, rtol=1e-8)
+        np.testing.assert_allclose(Hs, b / a, rtol=1e-9)
+        checked += 1
+    assert checked > 150
+    # L = 0 is no trial in both statistics
+    assert np.isnan(ot.trial(t, x, 3.0, 10, 0.2, 0, 10, dtep, stat="dchi2")[0])
+
+
+def test_detection_statistic_recovers_the_single_curve_period():
+    """The single-curve workload (trapezoid of depth 2e-3 at P = 3.7 d in
+    noise 1e-3): on every tenth period of the grid plus 3.7 d, the detection
+    statistic's best period is the injected one with H* near the depth;
+    tls.py's S/N picks another period (its value does not grow with the
+    number of transits in a window)."""
+    from gtrap_amd import synth
+    d = synth.tls_single_curve()
+    per = np.sort(np.concatenate([d["periods"][::10], [3.7]]))
+    r = ot.tlsfold(d["t"], d["flux"], per, d["widths"], 10, 0.0204, stat="dchi2")
+    best, val, sde = ot.tlsstats(r["sn"])
+    assert per[best] == 3.7 and val > 10 and sde > 5
+    assert abs(r["h"][best] + 2e-3) < 5e-4
+    r0 = ot.tlsfold(d["t"], d["flux"], per, d["widths"], 10, 0.0204)
+    assert per[ot.tlsstats(r0["sn"])[0]] != 3.7

@@ -36,7 +36,7 @@ def _call_ws(b, widths, nblocks):
     out = _outputs(b)
     _native.call("gtrap_tlsfold", *(p(o) for o in out), p(b.t), p(b.x), p(b.off), p(b.length),
                  b.nmax, p(b.periods), p(b.poff), b.ncurves, b.nptot, p(widths),
-                 int(widths.numel()), 10, 0.0204, p(ws), need, nblocks, _native.stream_handle())
+                 int(widths.numel()), 10, 0.0204, 0, p(ws), need, nblocks, _native.stream_handle())
     return out
 
 
@@ -46,7 +46,7 @@ def _call_static(b, widths, nblocks):
     out = _outputs(b)
     _native.call("gtrap_tlsfold_static", *(p(o) for o in out), p(b.t), p(b.x), p(b.off),
                  p(b.length), b.nmax, p(b.periods), p(b.poff), b.ncurves, b.nptot, p(widths),
-                 int(widths.numel()), 10, 0.0204, nblocks, _native.stream_handle())
+                 int(widths.numel()), 10, 0.0204, 0, nblocks, _native.stream_handle())
     return out
 
 
@@ -76,4 +76,4 @@ def test_static_workspace_rejects_oversized_curves(gpu):
     with pytest.raises(ValueError, match="static"):
         _native.call("gtrap_tlsfold_static", *(p(o) for o in out), p(b.t), p(b.x), p(b.off),
                      p(b.length), 5_000_000, p(b.periods), p(b.poff), b.ncurves, b.nptot,
-                     p(widths), 10, 10, 0.0204, 64, _native.stream_handle())
+                     p(widths), 10, 10, 0.0204, 0, 64, _native.stream_handle())
