@@ -89,6 +89,21 @@ def bench_gfilter(steps: int, warmup: int):
     wall = time.perf_counter() - t0
     kms = {r: float(np.mean([a.elapsed_time(b) for a, b in ev[r]])) for r in radii}
     log("gfilter kernel ms: " + ", ".join(f"r={r}: {kms[r]:.3f}" for r in radii))
+    # the wide windows of the reference's drivers (r = 100, 128, 512) on the
+    # first 1024 curves
+    wide = {}
+    for r in (100, 128, 512):
+        gfilter.medfilt_device(img, out, n, nq, r, "div", cols_out=1024)
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        gfilter.medfilt_device(img, out, n, nq, r, "div", cols_out=1024)
+        b.record()
+        torch.cuda.synchronize()
+        wide[str(r)] = dict(kernel_ms=a.elapsed_time(b), curves=1024,
+                            light_curves_per_s=1024 / (a.elapsed_time(b) * 1e-3))
+    log("gfilter wide windows (1024 curves): " +
+        ", ".join(f"r={k}: {v['kernel_ms']:.2f} ms" for k, v in wide.items()))
     bytes_per = 8.0 * n * nq
     dom = max(radii, key=lambda r: kms[r])
     ach = bytes_per / (kms[dom] * 1e-3) / 1e9
@@ -113,6 +128,7 @@ def bench_gfilter(steps: int, warmup: int):
                       frac=ach / HBM_PEAK_GBS, traffic=None,
                       kernel=f"medfilt_win_kernel r={dom}",
                       kernel_ms={str(r): kms[r] for r in radii}),
+        wide_windows=wide,
         cpu_baseline=dict(value=cpu_rate, unit="light curves/s", cores=1, kind="port",
                           sample=f"{nc} curves x 65536 samples, scipy.signal.medfilt r=7,24,64 "
                                  f"then divide, 1 thread ({model})"),

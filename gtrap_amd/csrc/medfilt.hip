@@ -165,6 +165,73 @@ __global__ __launch_bounds__(256) void medfilt_wide_kernel(
   y[(size_t)i * nq + q] = mode ? xv / med : xv - med;
 }
 
+// Wide windows in tiles: a workgroup filters rows [i0, i1) (at most
+// kWideTile) of one curve whose rows are x[i*stride] (i < n).  It stages
+// rows i0 - r .. i1 - 1 + r once into LDS as order-preserving keys (zero
+// outside [0, n), the reference's halo), then every thread finds its
+// window's (r+1)-th smallest key by a 32-step bisection over the key bits,
+// counting from LDS (consecutive threads read consecutive words).  The
+// result is the exact window median, as every selector returns the same
+// element.
+constexpr int kWideTile = 256;
+constexpr int kWideMaxR = 4096;  // LDS: (kWideTile + 2r) keys
+
+__device__ void medfilt_wide_tile(const float *__restrict__ x, float *__restrict__ y,
+                                  long stride, int n, int i0, int i1, int r, int mode,
+                                  uint32_t *keys) {
+  const int span = (i1 - i0) + 2 * r;
+  for (int k = threadIdx.x; k < span; k += blockDim.x) {
+    const int j = i0 - r + k;
+    keys[k] = fkey((j >= 0 && j < n) ? x[(long)j * stride] : 0.0f);
+  }
+  __syncthreads();
+  const int i = i0 + (int)threadIdx.x;
+  if (i < i1) {
+    const uint32_t *wk = keys + threadIdx.x;  // window of row i: wk[0 .. 2r]
+    uint32_t key = 0;  // smallest K with count(key <= K) >= r + 1
+    for (int bit = 31; bit >= 0; --bit) {
+      const uint32_t trial = key | ((1u << bit) - 1u);
+      int cnt = 0;
+      for (int j = 0; j <= 2 * r; ++j) cnt += (wk[j] <= trial) ? 1 : 0;
+      if (cnt < r + 1) key |= (1u << bit);
+    }
+    const float med = fkey_inv(key);
+    const float xv = x[(long)i * stride];
+    y[(long)i * stride] = mode ? xv / med : xv - med;
+  }
+  __syncthreads();
+}
+
+// dense image (interleaved (n, nq)): grid (ceil(rows_out / kWideTile), cols_out)
+__global__ __launch_bounds__(kWideTile) void medfilt_wide_tile_kernel(
+    const float *__restrict__ x, float *__restrict__ y, int n, int nq, int r, int mode,
+    int rows_out) {
+  extern __shared__ uint32_t wkeys[];
+  const int q = blockIdx.y;
+  const int i0 = blockIdx.x * kWideTile;
+  const int i1 = min(rows_out, i0 + kWideTile);
+  medfilt_wide_tile(x + q, y + q, nq, n, i0, i1, r, mode, wkeys);
+}
+
+// ragged segments: block b filters segment b (curve slot seg_curve[b], rows
+// seg_start[b] .. + seg) in tiles of kWideTile rows
+__global__ __launch_bounds__(kWideTile) void medfilt_ragged_wide_kernel(
+    const float *__restrict__ x, float *__restrict__ y, const int64_t *__restrict__ off,
+    const int32_t *__restrict__ len, const int32_t *__restrict__ seg_curve,
+    const int32_t *__restrict__ seg_start, const int32_t *__restrict__ nseg_dev, int r, int mode,
+    int seg) {
+  extern __shared__ uint32_t wkeys[];
+  const int b = blockIdx.x;
+  if (b >= *nseg_dev) return;
+  const int c = seg_curve[b];
+  const int n = len[c];
+  const int s0 = seg_start[b];
+  const int s1 = min(s0 + seg, n);
+  for (int i0 = s0; i0 < s1; i0 += kWideTile)
+    medfilt_wide_tile(x + off[c], y + off[c], 1, n, i0, min(s1, i0 + kWideTile), r, mode,
+                      wkeys);
+}
+
 // Per-curve masked mean over n (gtrap/gfilter.py:94-101): one block per curve,
 // float64 accumulation, fixed reduction tree (deterministic).
 __global__ __launch_bounds__(256) void offset_kernel(const float *__restrict__ img,
@@ -225,6 +292,13 @@ int medfilt_launch(const float *in, float *out, uint32_t n, uint32_t nq, uint32_
   GTRAP_REQUIRE(in != out, "gtrap_medfilt: in-place filtering is not supported");
   if (rows_out == 0 || cols_out == 0) return GTRAP_OK;
   const int w = 2 * (int)r + 1;
+  if (r > 64 && r <= (uint32_t)kWideMaxR && cols_out <= 65535) {
+    const size_t shm = (size_t)(kWideTile + 2 * r) * sizeof(uint32_t);
+    dim3 grid((rows_out + kWideTile - 1) / kWideTile, cols_out);
+    hipLaunchKernelGGL(medfilt_wide_tile_kernel, grid, dim3(kWideTile), shm, st, in, out, (int)n,
+                       (int)nq, (int)r, mode, (int)rows_out);
+    return check_launch("medfilt_wide_tile_kernel");
+  }
   if (r > 64) {
     const long total = (long)rows_out * cols_out;
     hipLaunchKernelGGL(medfilt_wide_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
@@ -283,9 +357,15 @@ int medfilt_ragged_launch(const float *x, float *y, const int64_t *off, const in
                           hipStream_t st) {
   GTRAP_REQUIRE(x && y && off && len && sc && ss && nseg, "gtrap_medfilt_ragged: null pointer");
   GTRAP_REQUIRE(mode == 0 || mode == 1, "gtrap_medfilt_ragged: mode must be 0 or 1");
-  GTRAP_REQUIRE(r <= 64, "gtrap_medfilt_ragged: r must be <= 64");
+  GTRAP_REQUIRE(r <= (uint32_t)kWideMaxR, "gtrap_medfilt_ragged: r must be <= %d", kWideMaxR);
   GTRAP_REQUIRE(seg >= 1 && max_segs >= 0, "gtrap_medfilt_ragged: bad segment shape");
   if (max_segs == 0) return GTRAP_OK;
+  if (r > 64) {
+    const size_t shm = (size_t)(kWideTile + 2 * r) * sizeof(uint32_t);
+    hipLaunchKernelGGL(medfilt_ragged_wide_kernel, dim3((unsigned)max_segs), dim3(kWideTile), shm,
+                       st, x, y, off, len, sc, ss, nseg, (int)r, mode, seg);
+    return check_launch("medfilt_ragged_wide_kernel");
+  }
   int rmax = 64;
   for (int b : kBuckets)
     if ((int)r <= b) { rmax = b; break; }

@@ -58,7 +58,9 @@ int gtrap_medfilt(const float *in, float *out, uint32_t n, uint32_t nq, uint32_t
  * c holds len[c] contiguous samples at x + off[c]; segment g filters rows
  * [seg_start[g], seg_start[g] + seg_len) of curve seg_curve[g] into y + off[c].
  * The segment count is read from *nseg_dev (<= max_segs) so the launch shape
- * is fixed and the call can be captured in a HIP graph.  r <= 64.
+ * is fixed and the call can be captured in a HIP graph.  r <= 4096 (r <= 64:
+ * sorted windows in registers; larger r: LDS tiles of 256 rows, the r the
+ * reference's drivers use: 100, 128, 512).
  */
 int gtrap_medfilt_ragged(const float *x, float *y, const int64_t *off, const int32_t *len,
                          const int32_t *seg_curve, const int32_t *seg_start,

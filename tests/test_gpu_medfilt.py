@@ -126,3 +126,22 @@ def test_detrend_image_full_size_subset(gpu):
         got = out[:, cols].cpu().numpy()
         want = om.detrend_scipy(sub, r, "div")
         np.testing.assert_array_equal(_bits(got), _bits(want), err_msg=f"r={r}")
+
+
+@pytest.mark.parametrize("r", [65, 100, 128, 512])
+def test_wide_windows_of_the_reference_drivers(gpu, r):
+    """The r of kepler/gtls_mockkepler.py:229 (128), kepler/gtls_pickkepler.py:197
+    (100) and works/sp/gtls_tesslc.py:129 (512), on LDS tiles: bit-exact
+    against scipy.signal.medfilt, sub and div, with outliers and edges."""
+    import torch
+    from gtrap_amd import gfilter
+    rng = np.random.default_rng(r)
+    n, nq = 3000, 5
+    img = (1.0 + 1e-3 * rng.standard_normal((n, nq))).astype(np.float32)
+    img[rng.random((n, nq)) < 0.01] += np.float32(1e-2)
+    x = torch.from_numpy(img).cuda()
+    for mode in ("sub", "div"):
+        y = torch.empty_like(x)
+        gfilter.medfilt_device(x, y, n, nq, r, mode)
+        want = om.detrend_scipy(img, r, mode)
+        np.testing.assert_array_equal(y.cpu().numpy().view(np.uint32), want.view(np.uint32))

@@ -68,3 +68,17 @@ def test_admission_by_budget(gpu):
         peak = max(peak, len(srv.slots))
         assert srv.salloc.used <= srv.scap and srv.palloc.used <= srv.pcap
     assert len(res) == 6 and peak < 6
+
+
+def test_served_equals_plain_wide_detrend(gpu):
+    """The drivers' wide windows (r = 128, kepler/gtls_mockkepler.py:229) in the
+    ragged detrend of the serving loop."""
+    from gtrap_amd.serve import ServerConfig, TransitServer, search_plain
+    reqs = _requests(k=5, seed=3)
+    cfg = ServerConfig(items_per_step=131, max_curves=8, max_len=9000, hbm_budget=64 << 20,
+                       nblocks=256, r=128)
+    plain = {r.rid: search_plain(r, cfg.widths, cfg.nl, cfg.dtep, cfg.r) for r in reqs}
+    got = TransitServer(cfg).run(_requests(k=5, seed=3), realtime=False)
+    assert len(got) == 5
+    for g in got:
+        _same(g, plain[g.rid])
