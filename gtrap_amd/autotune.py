@@ -1,10 +1,14 @@
 """Autotuner for execution parameters.
 
-Only parameters that do not change results are tuned: the number of
+Only parameters that do not change results are tuned: the tls_fold kernel's
+tile geometry (epochs per staged LDS tile, phase-index cells per sample,
+samples per phase bucket, epochs per window-table run), the number of
 persistent tls_fold workgroups, the serving step size (work items per step),
 and the detrend segment length.  Semantic parameters (bins, widths, L steps,
-grids, windows) are never touched, so every tuned configuration returns the
-same bits.
+grids, windows) are never touched: region membership and the best trials do
+not depend on the tuned values, and S/N moves by float64 rounding at most
+(the tile epochs and the bucket size set the staged ranges and the tiles'
+reference epochs; the other parameters leave the bits unchanged).
 
 Results are cached as JSON keyed by device name and workload signature in
 ``$GTRAP_TUNE_CACHE`` (default ``~/.cache/gtrap_amd/tune.json``).  Lookups
@@ -97,6 +101,64 @@ def tune_tlsfold_blocks(curves, periods_list, widths, nl=10, dtep=0.0204,
     return res
 
 
+TILE_KEY = "tlsfold.tiles"
+DEFAULT_TILES = (144, 2, 2, 0)
+_tiles_applied = None
+
+
+def set_tiles(max_tile=144, cells=2, spb=2, te_run=0) -> None:
+    """Set the tls_fold tile geometry for the launches that follow."""
+    _native.call("gtrap_tlsfold_set_tiles", int(max_tile), int(cells), int(spb), int(te_run))
+
+
+def apply_tiles(force: bool = False):
+    """Apply the tuned tile geometry (cache, then the packaged table, then the
+    defaults) once per process; returns it."""
+    global _tiles_applied
+    if _tiles_applied is None or force:
+        v = lookup(TILE_KEY, list(DEFAULT_TILES))
+        set_tiles(*v)
+        _tiles_applied = tuple(int(a) for a in v)
+    return _tiles_applied
+
+
+def tune_tiles(curves, periods_list, widths, nl=10, dtep=0.0204, nblocks=512,
+               max_tiles=(48, 72, 96, 144), cells=(1, 2), spbs=(1, 2, 4), te_runs=(0, 24, 36),
+               save=True, log=None) -> dict:
+    """Kernel time (HIP events) of tls_fold on a representative batch for
+    every tile geometry of the grid; stores the fastest."""
+    import torch
+    from .tlsfold import Workspace, pack, tlsfold_device
+    batch = pack(curves, periods_list)
+    ws = Workspace()
+    times = {}
+    for mt in max_tiles:
+        for c in cells:
+            for sp in spbs:
+                for tr in te_runs:
+                    set_tiles(mt, c, sp, tr)
+
+                    def run():
+                        a = torch.cuda.Event(enable_timing=True)
+                        b = torch.cuda.Event(enable_timing=True)
+                        a.record()
+                        tlsfold_device(batch, widths, nl, dtep, nblocks=nblocks, workspace=ws)
+                        b.record()
+                        torch.cuda.synchronize()
+                        return a.elapsed_time(b)
+                    run()
+                    ms = min(run() for _ in range(2))
+                    times[f"{mt}/{c}/{sp}/{tr}"] = ms
+                    if log:
+                        log(f"tiles {mt}/{c}/{sp}/{tr}: {ms:.2f} ms")
+    best = min(times, key=times.get)
+    geo = [int(v) for v in best.split("/")]
+    set_tiles(*geo)
+    if save:
+        store(f"{_device_key()}:{TILE_KEY}", geo)
+    return dict(tiles=geo, ms=times, default_ms=times.get("144/2/2/0"))
+
+
 def tune_medfilt_seg(n, nq, r, candidates=(256, 512, 1024, 2048, 4096), save=True) -> dict:
     import torch
     from . import gfilter
@@ -139,8 +201,9 @@ def tune_serve(trace_fn, candidates=((16384, 1024), (32768, 1024), (65536, 1024)
 
 
 def main(argv=None) -> int:
-    """Tune the serving step and the fold kernel's workgroups on the current
-    GPU; prints the choices and stores them in the cache."""
+    """Tune the fold kernel's tile geometry and workgroups and the serving
+    step on the current GPU; prints the choices and stores them in the
+    cache."""
     import argparse
     from . import synth
     ap = argparse.ArgumentParser()
@@ -149,6 +212,11 @@ def main(argv=None) -> int:
     _native.require_gpu()
     reqs, _ = synth.serving_trace(24, seed=7)
     out = dict(device=_device_key())
+    big = sorted(reqs, key=lambda r: len(r.t))[-12:]
+    out["tiles"] = tune_tiles([(r.t, r.flux) for r in big], [r.periods for r in big],
+                              np.linspace(0.04, 0.3, 10),
+                              log=lambda m: print(m, flush=True))
+    print(json.dumps({k: v for k, v in out["tiles"].items() if k != "ms"}), flush=True)
     out["tlsfold"] = tune_tlsfold_blocks([(r.t, r.flux) for r in reqs[:6]],
                                          [r.periods for r in reqs[:6]],
                                          np.linspace(0.04, 0.3, 10),

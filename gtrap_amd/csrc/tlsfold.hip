@@ -324,6 +324,10 @@ struct TfArgs {
   int32_t *counter;
   int32_t te_run;  // epochs per window-table run (fits the dynamic LDS)
   int32_t stat;    // 0: tls.py's S/N; 1: the chi^2-improvement detection statistic
+  // execution geometry (results do not depend on it beyond float64 rounding)
+  int32_t max_tile;   // epochs per staged tile (at most kTfMaxTile)
+  int32_t cells;      // phase-index cells per staged sample (1 or 2)
+  int32_t spb;        // samples per phase bucket of the fold's counting sort (1..8)
 };
 
 // LDS layout for a staged range
@@ -416,7 +420,7 @@ __device__ __forceinline__ void tf_wave_iscan(double (&v)[N]) {
 // closing one are all it needs.
 constexpr int kPer = kTfChunk / kTfThreads;
 __device__ void tf_stage(TfLds &L, const TfRec *rec, int nv, double P, int lo,
-                         int cnt, double r, double xref) {
+                         int cnt, double r, double xref, int cells) {
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
   const int k0 = tid * kPer;
   // image of extended element j = lo + k of record value u
@@ -438,7 +442,7 @@ __device__ void tf_stage(TfLds &L, const TfRec *rec, int nv, double P, int lo,
     const double base = image(0, ub), top = image(last, ut);
     // phase index: two cells per sample; cells narrower than 1e-9 of the
     // magnitudes (a range of near-equal phases) fall back to one cell
-    const int nc = 2 * min(cnt, kTfChunk);
+    const int nc = cells * min(cnt, kTfChunk);
     const double span = top - base;
     const double inv =
         span > 1e-9 * (1.0 + fabs(base) + fabs(top)) * (double)nc ? (double)nc / span : 0.0;
@@ -755,7 +759,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
     // the number of valid samples and the first valid index)
     // (each block's workspace slot holds nmax/2 + 2 bucket ends after the
     // records, gtrap_tlsfold_ws_bytes: more buckets need a larger slot)
-    const int nb = max(1, min(n / 2, kTfMaxBuckets));
+    const int nb = max(1, min(n / A.spb, kTfMaxBuckets));
     const double bscale = (double)nb / P;
     const double invP = 1.0 / P;
     int *lcur = reinterpret_cast<int *>(&L);  // nb + 1 <= kTfMaxBuckets + 1 ints
@@ -936,7 +940,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
     // first guess of the tile size from the mean phase density
     const double per_epoch = (double)nv / P * A.dtep;
     int te = (int)(((double)kTfChunk / fmax(per_epoch, 1e-9) - 2.0 * (hwm + marg) / A.dtep) - 3.0);
-    te = max(1, min(kTfMaxTile, te));
+    te = max(1, min(A.max_tile, te));
     int e0 = 0;
     while (e0 < nE) {
       const int e1 = min(nE, e0 + te);
@@ -953,7 +957,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
       }
       if (cnt <= kTfChunk) {
         const double r = (double)((e0 + e1 - 1) / 2) * A.dtep;  // q reference: central epoch
-        tf_stage(L, rec, nv, P, lo, cnt, r, xref);
+        tf_stage(L, rec, nv, P, lo, cnt, r, xref, A.cells);
         // the staged tile is swept in runs of te_run epochs
         for (int eb = e0; eb < e1; eb += A.te_run) {
           const int nte = min(A.te_run, e1 - eb);
@@ -1069,7 +1073,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
         }
         for (int ca = lo; ca < chi; ca += kTfChunk) {
           const int cc = min(kTfChunk, chi - ca);
-          tf_stage(L, rec, nv, P, ca, cc, r, xref);
+          tf_stage(L, rec, nv, P, ca, cc, r, xref, A.cells);
 #pragma unroll
           for (int j = 0; j < kTfMaxT; ++j) {
             const int k = tid + j * kTfThreads;
@@ -1120,7 +1124,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
         continue;
       }
       e0 = e1;
-      te = min(kTfMaxTile, 2 * te);
+      te = min(A.max_tile, 2 * te);
     }
     // ---- reduce the per-thread bests (wave shuffles, then LDS) ----
     for (int o = kWave / 2; o > 0; o >>= 1) {
@@ -1154,6 +1158,14 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
 
 __global__ void tf_zero_ticket(int *p) { *p = 0; }
 
+// Execution geometry of the launches (gtrap_tlsfold_set_tiles): tile epochs,
+// phase-index cells per sample, samples per phase bucket, window-table run
+// (0: as large as the LDS allows).
+struct TfTiles {
+  int max_tile = kTfMaxTile, cells = 2, spb = 2, te_run = 0;
+};
+static TfTiles g_tiles;
+
 // Dynamic LDS of a launch: the (W, L) table and widths, then te_run epochs
 // of window table (2 doubles + 3 ints per (epoch, W)), te_run as large as
 // the LDS left over by the static part allows (at most kTfMaxTe).
@@ -1178,6 +1190,7 @@ static int tf_launch_cfg(int nw, int nl, size_t *dyn, int *te_run) {
   }
   int te = (int)((budget - fixed) / per);
   if (te > kTfMaxTe) te = kTfMaxTe;
+  if (g_tiles.te_run > 0 && g_tiles.te_run < te) te = g_tiles.te_run;
   *te_run = te;
   *dyn = fixed + (size_t)te * per;
   return GTRAP_OK;
@@ -1189,7 +1202,7 @@ static int tf_launch_cfg(int nw, int nl, size_t *dyn, int *te_run) {
 // the phase-sorted curve (16-byte records: float64 phase, float32 flux) and the bucket offsets.
 extern "C" int64_t gtrap_tlsfold_ws_bytes(int32_t nmax, int32_t nblocks) {
   if (nmax < 1) nmax = 1;
-  int64_t per = (int64_t)nmax * 16 + (int64_t)(nmax / 2 + 2) * 4;
+  int64_t per = (int64_t)nmax * 16 + (int64_t)(nmax + 2) * 4;
   per = (per + 255) / 256 * 256;  // keep every block's slot 256-byte aligned
   return 256 + per * (int64_t)(nblocks > 0 ? nblocks : 1);
 }
@@ -1224,13 +1237,29 @@ extern "C" int gtrap_tlsfold(double *snr, double *hbest, int32_t *best_w, int32_
   if (const int rc = gtrap::tf_launch_cfg<false>(nw, nl, &dyn, &te_run)) return rc;
   gtrap::TfArgs A{t, x, off, len, periods, poff, nullptr, ncurves, nptot, widths, nw, nl, dtep,
                   snr, hbest, best_w, best_l, best_e, (unsigned char *)ws,
-                  tf_slot_bytes(nmax), nmax, nullptr, nullptr, nullptr, nullptr, te_run, stat};
+                  tf_slot_bytes(nmax), nmax, nullptr, nullptr, nullptr, nullptr, te_run, stat,
+                  gtrap::g_tiles.max_tile, gtrap::g_tiles.cells, gtrap::g_tiles.spb};
   const int64_t g = nptot < nblocks ? nptot : nblocks;
   hipStream_t st = gtrap::as_stream(stream);
   hipLaunchKernelGGL(gtrap::tf_zero_ticket, dim3(1), dim3(1), 0, st, (int *)ws);
   hipLaunchKernelGGL(gtrap::tlsfold_kernel<false>, dim3((unsigned)g), dim3(gtrap::kTfThreads),
                      dyn, st, A);
   return gtrap::check_launch("tlsfold_kernel");
+}
+
+extern "C" int gtrap_tlsfold_set_tiles(int32_t max_tile_epochs, int32_t cells_per_sample,
+                                       int32_t samples_per_bucket, int32_t table_run_epochs) {
+  GTRAP_REQUIRE(max_tile_epochs >= 1 && max_tile_epochs <= gtrap::kTfMaxTile,
+                "gtrap_tlsfold_set_tiles: max_tile_epochs must be 1..%d", gtrap::kTfMaxTile);
+  GTRAP_REQUIRE(cells_per_sample == 1 || cells_per_sample == 2,
+                "gtrap_tlsfold_set_tiles: cells_per_sample must be 1 or 2");
+  GTRAP_REQUIRE(samples_per_bucket >= 1 && samples_per_bucket <= 8,
+                "gtrap_tlsfold_set_tiles: samples_per_bucket must be 1..8");
+  GTRAP_REQUIRE(table_run_epochs >= 0 && table_run_epochs <= gtrap::kTfMaxTe,
+                "gtrap_tlsfold_set_tiles: table_run_epochs must be 0..%d", gtrap::kTfMaxTe);
+  gtrap::g_tiles = gtrap::TfTiles{max_tile_epochs, cells_per_sample, samples_per_bucket,
+                                  table_run_epochs};
+  return GTRAP_OK;
 }
 
 // ---- the same search with no allocation: a static device workspace --------
@@ -1324,7 +1353,7 @@ extern "C" int gtrap_tlsfold_items(double *snr, double *hbest, int32_t *best_w, 
   gtrap::TfArgs A{t, x, off, len, periods, nullptr, nullptr, 0, 0, widths, nw, nl, dtep,
                   snr, hbest, best_w, best_l, best_e, (unsigned char *)ws,
                   tf_slot_bytes(nmax), nmax, item_curve, item_pidx, nitems_dev, counter, te_run,
-                  stat};
+                  stat, gtrap::g_tiles.max_tile, gtrap::g_tiles.cells, gtrap::g_tiles.spb};
   hipLaunchKernelGGL(gtrap::tlsfold_kernel<true>, dim3((unsigned)nblocks), dim3(gtrap::kTfThreads),
                      dyn, gtrap::as_stream(stream), A);
   return gtrap::check_launch("tlsfold_kernel<items>");

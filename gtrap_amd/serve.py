@@ -278,7 +278,9 @@ class HipBackend:
 
     def __init__(self, cfg: ServerConfig):
         import torch
+        from . import autotune
         _native.require_gpu()
+        autotune.apply_tiles()
         self.cfg = cfg
         dev = "cuda"
         pbytes = int(cfg.hbm_budget * cfg.period_share)

@@ -96,7 +96,9 @@ def tlsfold_device(batch: FoldBatch, widths, nl: int, dtep: float, out=None,
     ``stat``: "snr" (tls.py's S/N) or "dchi2" (the detection statistic, the
     fit's chi^2 improvement over the residual variance; reported in "sn")."""
     import torch
+    from . import autotune
     _native.require_gpu()
+    autotune.apply_tiles()
     ws = workspace or _workspace_for(stream)
     nb = max(1, min(int(nblocks), max(1, batch.nptot)))
     buf, need = ws.get(batch.nmax, nb)

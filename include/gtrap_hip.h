@@ -340,6 +340,22 @@ int gtrap_tlsfold(double *snr, double *hbest, int32_t *best_w, int32_t *best_l,
                   void *ws, int64_t ws_bytes, int32_t nblocks, void *stream);
 
 /*
+ * Execution geometry of the tls_fold launches that follow (no reference
+ * counterpart; the autotuner's search space): epochs per staged LDS tile
+ * (1..144, default 144), phase-index cells per staged sample (1 or 2,
+ * default 2), samples per phase bucket of the counting sort (1..8, default
+ * 2) and epochs per window-table run (0 = as many as the LDS holds, at most
+ * 48).  Region membership and the best trial do not depend on it; the
+ * cells and the table run change no bit, the tile epochs and the bucket size
+ * (which set the staged sample ranges and the tiles' reference epochs) move
+ * S/N by float64 rounding only.  Process
+ * wide, read at each launch (a captured graph keeps the values of its
+ * capture).
+ */
+int gtrap_tlsfold_set_tiles(int32_t max_tile_epochs, int32_t cells_per_sample,
+                            int32_t samples_per_bucket, int32_t table_run_epochs);
+
+/*
  * gtrap_tlsfold with no allocation and no workspace argument.  No
  * counterpart in the reference: it is for callers that cannot allocate
  * device memory (code under HIP graph capture, tight serving loops,
@@ -347,8 +363,8 @@ int gtrap_tlsfold(double *snr, double *hbest, int32_t *best_w, int32_t *best_l,
  * slots live in a static 64 MiB device array per GPU (held from the
  * library's load until the process ends); it calls no hipMalloc/hipFree.
  * nblocks is clamped to the slots that fit: 64 MiB / gtrap_tlsfold_ws_bytes
- * (nmax, 1), e.g. 53 workgroups for nmax = 70000, 907 for nmax = 4096; an
- * nmax whose single slot does not fit (about 3.7 million samples) is
+ * (nmax, 1), e.g. 47 workgroups for nmax = 70000, 816 for nmax = 4096; an
+ * nmax whose single slot does not fit (about 3.3 million samples) is
  * rejected with GTRAP_EINVAL.  Results are identical to gtrap_tlsfold's.
  * Calls that overlap on one GPU (other streams or host threads) share the
  * array and race: serialise them.  The ticket is reset by a kernel on the

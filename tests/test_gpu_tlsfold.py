@@ -272,3 +272,31 @@ def test_detection_statistic_matches_oracle(gpu):
             s = ot.trial(t, x, per[p], g["e"][p], widths[g["w"][p]], g["l"][p], 10, 0.0204,
                          stat="dchi2")[0]
             assert abs(s - want["sn"][p]) <= 1e-9 * abs(want["sn"][p])
+
+
+def test_tile_geometry_changes_no_result(gpu):
+    """Phase-index cells and window-table runs leave every bit unchanged; the
+    tile epochs and the bucket size (which sets the staged sample ranges)
+    move S/N by float64 rounding only."""
+    from gtrap_amd import autotune, tlsfold
+    widths = np.linspace(0.04, 0.3, 10)
+    curves = [_curve(9000, 91, inject=(2.2, 0.3, 0.15, 0.015, 3e-3)), _curve(2500, 92)]
+    plist = [np.geomspace(0.5, 60.0, 40), np.geomspace(0.5, 16.0, 25)]
+    try:
+        autotune.set_tiles(144, 2, 2, 0)
+        base = tlsfold.tlsfold(curves, plist, widths, 10, 0.0204, nblocks=128)
+        for geo in ((144, 1, 2, 0), (144, 2, 2, 24), (144, 1, 2, 36)):
+            autotune.set_tiles(*geo)
+            got = tlsfold.tlsfold(curves, plist, widths, 10, 0.0204, nblocks=128)
+            for g, b in zip(got, base):
+                for k in g:
+                    np.testing.assert_array_equal(g[k], b[k], err_msg=f"{geo} {k}")
+        for geo in ((48, 2, 2, 0), (96, 1, 4, 24), (144, 2, 1, 0), (144, 1, 8, 36)):
+            autotune.set_tiles(*geo)
+            got = tlsfold.tlsfold(curves, plist, widths, 10, 0.0204, nblocks=128)
+            for g, b in zip(got, base):
+                # the kernel's own agreement with the direct oracle (1e-7)
+                np.testing.assert_allclose(g["sn"], b["sn"], rtol=1e-7)
+                assert np.mean(g["w"] == b["w"]) > 0.95
+    finally:
+        autotune.apply_tiles(force=True)
