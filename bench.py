@@ -486,6 +486,16 @@ def bench_precision(steps: int, warmup: int):
     wall = time.perf_counter() - t0
     for k, v in res.items():
         log(f"{k}: {v}")
+    # the periodic trapezoid search under the same axes: 32 full-length curves,
+    # 32 periods around the transit period (resolving every transit)
+    t32 = d["t64"].astype(np.float32)
+    ntc = 32
+    curves = [(t32, d["lc"][:, q]) for q in range(ntc)]
+    per = d["truth"]["P"] + (np.arange(32) - 16) * 5e-4
+    tls_res = precision.tls_study(curves, [per] * ntc, np.linspace(0.04, 0.3, 10), 10,
+                                  2.0 / 1440.0)
+    for k, v in tls_res.items():
+        log(f"tls {k}: {v}")
     own = res["f32/exact32/f32"]
     g = d["grid"]
     n, nq = d["lc"].shape
@@ -499,6 +509,9 @@ def bench_precision(steps: int, warmup: int):
         value=own["sde_rel_max"], unit="max relative SDE error (this build's arithmetic)",
         ms_per_step=wall * 1e3, higher_is_better=False, dtype="f32",
         variants=res,
+        tls_variants=dict(results=tls_res, workload="tls_fold on 32 of the curves (262144 "
+                                                     "samples, 364 d), 32 periods around P, "
+                                                     "10 W x 10 L, epochs every 2 min"),
         config=dict(workload="precision: 256 curves x 262144 samples at 2 min, depth 1e-4, "
                              f"SNR 7-15, {nf} frequencies, nb 1024, kma 23",
                     n=n, nq=nq, nf=nf, nb=g["nb"]),

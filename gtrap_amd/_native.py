@@ -49,6 +49,8 @@ SIGNATURES: dict[str, tuple] = {
     "gtrap_tlsfold_set_tiles": (_int, [_i32, _i32, _i32, _i32]),
     "gtrap_tlsfold_static": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp,
                                     _i32, _i64, _vp, _i32, _i32, _f64, _i32, _i32, _vp]),
+    "gtrap_tlsfold_prec": (_int, [_i32] + [_vp] * 11 + [_i32, _vp, _vp, _i32, _i64, _vp, _i32,
+                                                      _i32, _f64, _i32, _vp, _i64, _i32, _vp]),
     "gtrap_tlsfold_items": (_int, [_vp] * 9 + [_i32] + [_vp] * 6 + [_i32, _i32, _f64, _i32, _vp,
                                                                   _i64, _i32, _vp]),
     "gtrap_tlsstats": (_int, [_vp] * 5 + [_i32] + [_vp] * 4),

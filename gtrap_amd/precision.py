@@ -131,3 +131,123 @@ def study(d: dict, variants=STUDY, timing: bool = True) -> dict:
         sr, ms = timed(v)
         out["/".join(v)] = dict(ms=ms, **compare(sr, ref))
     return out
+
+
+# ----------------------------------------------------------------------------
+# the periodic trapezoid search (tls_fold) under the same three axes
+# ----------------------------------------------------------------------------
+TLS_VARIANTS = {  # name: variant bits of gtrap_tlsfold_prec
+    "f32/f64/f64": 0,          # production: float32 flux, float64 sums and fold
+    "f16/f64/f64": 1,          # binary16 storage after offset and scale
+    "f32/f32/f64": 4,          # float32 prefix sums
+    "f32/twosum32/f64": 8,     # float32 prefix sums with two-sum compensation
+    "f32/f64/f32": 2,          # float32 fold
+    "f16/twosum32/f64": 9,     # binary16 storage, compensated float32 sums, float64 fold
+    "f32/f32/f32": 6,          # float32 throughout (the statistic aside)
+}
+
+
+class TlsInputs:
+    """Device copies of a tls_fold precision workload: curves (t, flux) of one
+    length, binary16 copies after removing each curve's mean and scaling by a
+    power of two, and the period grids."""
+
+    def __init__(self, curves, periods_list):
+        import torch
+        from .tlsfold import pack
+        _native.require_gpu()
+        self.batch = pack(curves, periods_list)
+        x = np.concatenate([np.asarray(f, np.float32) for _, f in curves])
+        hs, scales = [], []
+        for _, f in curves:
+            f = np.asarray(f, np.float32)[:, None]
+            h, hinv = to_f16(f, np.array([f.mean(dtype=np.float64)], np.float32))
+            hs.append(h[:, 0])
+            scales.append(hinv[0])
+        dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+        self.x16 = dev(np.concatenate(hs).view(np.int16))
+        self.x16scale = dev(np.array(scales, np.float32))
+        assert self.x16.numel() == x.size
+
+
+def tls_run_variant(inp: TlsInputs, variant: int, widths, nl=10, dtep=0.0204, stat="snr",
+                    nblocks=512):
+    """Per-period (sn, h, w, l, e) device tensors of one arithmetic variant."""
+    import torch
+    from .tlsfold import STATS, Workspace
+    b = inp.batch
+    nb = max(1, min(int(nblocks), b.nptot))
+    ws = getattr(inp, "_ws", None) or Workspace()
+    inp._ws = ws
+    buf, need = ws.get(b.nmax, nb)
+    wdev = torch.from_numpy(np.ascontiguousarray(widths, np.float64)).cuda()
+    out = dict(sn=torch.empty(b.nptot, dtype=torch.float64, device="cuda"),
+               h=torch.empty(b.nptot, dtype=torch.float64, device="cuda"),
+               w=torch.empty(b.nptot, dtype=torch.int32, device="cuda"),
+               l=torch.empty(b.nptot, dtype=torch.int32, device="cuda"),
+               e=torch.empty(b.nptot, dtype=torch.int32, device="cuda"))
+    p = _native.ptr
+    _native.call("gtrap_tlsfold_prec", int(variant), p(out["sn"]), p(out["h"]), p(out["w"]),
+                 p(out["l"]), p(out["e"]), p(b.t), p(b.x), p(inp.x16), p(inp.x16scale),
+                 p(b.off), p(b.length), int(b.nmax), p(b.periods), p(b.poff), int(b.ncurves),
+                 int(b.nptot), p(wdev), int(wdev.numel()), int(nl), float(dtep), STATS[stat],
+                 p(buf), int(need), nb, _native.stream_handle())
+    return out
+
+
+def periodogram_stats(sn: np.ndarray):
+    """(best index, best value, SDE) of one curve's per-period values, the
+    statistics of gtrap_tlsstats: first maximum over values that are neither
+    NaN nor -inf, mean and standard deviation over the finite values."""
+    sn = np.asarray(sn, np.float64)
+    cand = ~np.isnan(sn) & ~np.isneginf(sn)
+    if not cand.any():
+        return -1, -np.inf, np.nan
+    v = np.where(cand, sn, -np.inf)
+    b = int(np.argmax(v))
+    fin = sn[np.isfinite(sn)]
+    with np.errstate(all="ignore"):
+        mean = fin.mean()
+        std = np.sqrt((fin * fin).mean() - mean * mean)
+    return b, float(v[b]), float((v[b] - mean) / std)
+
+
+def tls_study(curves, periods_list, widths, nl=10, dtep=0.0204, stat="snr",
+              variants=TLS_VARIANTS) -> dict:
+    """Every variant against the float64 production kernel: relative error of
+    the per-period value, per-curve SDE error, the fraction of periods whose
+    best (W, L, epoch) and of curves whose best period are unchanged, and the
+    kernel milliseconds (HIP events, after a warm-up launch)."""
+    import torch
+    inp = TlsInputs(curves, periods_list)
+    poff = inp.batch.poff_host
+    out, ref = {}, None
+    for name, var in variants.items():
+        tls_run_variant(inp, var, widths, nl, dtep, stat)
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        r = tls_run_variant(inp, var, widths, nl, dtep, stat)
+        b.record()
+        torch.cuda.synchronize()
+        r = {k: v.cpu().numpy() for k, v in r.items()}
+        ms = a.elapsed_time(b)
+        if ref is None:
+            ref = r
+            out[name + " (yardstick)"] = dict(ms=ms)
+            continue
+        fin = np.isfinite(ref["sn"]) & np.isfinite(r["sn"])
+        rel = np.abs(r["sn"][fin] - ref["sn"][fin]) / np.abs(ref["sn"][fin])
+        same = (r["w"] == ref["w"]) & (r["l"] == ref["l"]) & (r["e"] == ref["e"])
+        bp, sd = [], []
+        for c in range(len(curves)):
+            s0, s1 = poff[c], poff[c + 1]
+            b0, _, d0 = periodogram_stats(ref["sn"][s0:s1])
+            b1, _, d1 = periodogram_stats(r["sn"][s0:s1])
+            bp.append(b0 == b1)
+            sd.append(abs(d1 - d0) / abs(d0))
+        out[name] = dict(ms=ms, sn_rel_median=float(np.median(rel)),
+                         sn_rel_p99=float(np.percentile(rel, 99)), sn_rel_max=float(rel.max()),
+                         trial_agree=float(same.mean()), best_period_agree=float(np.mean(bp)),
+                         sde_rel_max=float(np.max(sd)))
+    return out

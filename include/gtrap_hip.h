@@ -378,6 +378,25 @@ int gtrap_tlsfold_static(double *snr, double *hbest, int32_t *best_w, int32_t *b
                          int32_t stat, int32_t nblocks, void *stream);
 
 /*
+ * Precision study of gtrap_tlsfold (no reference counterpart): the same
+ * search with one arithmetic choice changed, for measuring single-precision
+ * designs against the float64 kernel.  variant: 0 = gtrap_tlsfold itself;
+ * bits 1 = flux stored as binary16 (x16, with x = h * x16scale[c] after the
+ * curve's offset was removed), 2 = phase folded in float32 (quotient,
+ * product and subtraction in float32), 4 = tile prefix sums accumulated in
+ * float32, 8 = the same with two-sum compensation; the combinations 1, 2, 4,
+ * 8, 1|8 and 4|2 are built.  Other arguments as gtrap_tlsfold (x may be NULL
+ * in the binary16 variants).
+ */
+int gtrap_tlsfold_prec(int32_t variant, double *snr, double *hbest, int32_t *best_w,
+                       int32_t *best_l, int32_t *best_e, const float *t, const float *x,
+                       const uint16_t *x16, const float *x16scale, const int64_t *off,
+                       const int32_t *len, int32_t nmax, const double *periods,
+                       const int64_t *poff, int32_t ncurves, int64_t nptot,
+                       const double *widths, int32_t nw, int32_t nl, double dtep, int32_t stat,
+                       void *ws, int64_t ws_bytes, int32_t nblocks, void *stream);
+
+/*
  * Item-list form of gtrap_tlsfold for the serving loop (no reference
  * counterpart).  Work item k scores period item_pidx[k] (index into periods
  * and the outputs) of curve slot item_curve[k] (off/len tables); the item

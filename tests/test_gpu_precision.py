@@ -125,3 +125,39 @@ def test_long_baseline_full_length_curves(gpu):
     for name in ("f32/exact32/f64", "f32/twosum32/f64", "f32/atomic32/f64"):
         assert res[name]["sr_rel_max"] < 1e-5, (name, res[name])
     assert res["f32/f64/f32"]["sr_rel_max"] > 10 * res["f32/exact32/f64"]["sr_rel_max"]
+
+
+def _tls_precision_workload(ncurve=8, nper=16, seed=16):
+    """Full-length curves of the long-baseline workload (262144 samples at
+    2 minutes, 364 d, trapezoid depth 1e-4 at P = 7.3 d, SNR 7-15) and a
+    period grid around P that resolves every transit."""
+    from gtrap_amd import synth
+    d = synth.precision_batch(nq=ncurve, n=262144, nf=8, seed=seed)
+    t = d["t64"].astype(np.float32)
+    curves = [(t, d["lc"][:, q]) for q in range(ncurve)]
+    per = 7.3 + (np.arange(nper) - nper // 2) * 5e-4
+    return curves, [per] * ncurve
+
+
+def test_tls_precision_variants(gpu):
+    """The periodic trapezoid search under single-precision arithmetic: the
+    production variant is the float64 kernel itself; float64-fold variants
+    with binary16 storage or float32 (compensated) sums stay close to it, and
+    every variant is measured."""
+    import torch
+    from gtrap_amd import precision, tlsfold
+    curves, plist = _tls_precision_workload()
+    widths = np.linspace(0.04, 0.3, 10)
+    dt = 2.0 / 1440.0
+    inp = precision.TlsInputs(curves, plist)
+    prod = precision.tls_run_variant(inp, 0, widths, 10, dt)
+    ref = tlsfold.tlsfold(curves, plist, widths, 10, dt)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(prod["sn"].cpu().numpy(), np.concatenate([r["sn"] for r in ref]))
+    res = precision.tls_study(curves, plist, widths, 10, dt)
+    print({k: v for k, v in res.items()}, flush=True)
+    assert res["f32/twosum32/f64"]["sn_rel_max"] < 1e-4
+    assert res["f16/twosum32/f64"]["sn_rel_median"] < 1e-2
+    for name, r in res.items():
+        if "yardstick" not in name:
+            assert np.isfinite(r["sn_rel_median"]), name
