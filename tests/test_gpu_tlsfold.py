@@ -60,6 +60,17 @@ def test_chunked_windows_small_periods(gpu):
     _check(curves, plist, widths, 6)
 
 
+def test_periods_below_the_widest_window(gpu):
+    """Periods below the widest window: one epoch's range holds the folded
+    curve more than twice (chunked sweep over images of both neighbouring
+    cycles); the widths at or above the period give no trial, those below
+    it do."""
+    widths = np.array([0.05, 0.1, 0.3])
+    curves = [_curve(12000, 41, inject=(0.13, 0.05, 0.04, 0.004, 3e-3))]
+    plist = [np.array([0.11, 0.12, 0.13, 0.14])]
+    _check(curves, plist, widths, 5)
+
+
 def test_single_epoch_limit_and_nl_grid(gpu):
     widths = np.array([0.05, 0.2])
     t, x = _curve(400, 5, mask=0.0)
