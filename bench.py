@@ -31,9 +31,9 @@ TRIAL_FLOPS = 103
 # measured offline: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch of
 # tlsfold_kernel<true>, scaled by the same counters over a 1 GiB copy
 # (factors 2.0 and 1.0)
-SERVE_FOLD_TRAFFIC = 1.317e11
+SERVE_FOLD_TRAFFIC = 1.212e11
 SERVE_FOLD_TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this workload "
-                             "before the current build (not measured in this run), "
+                             "on this build (separate runs, not measured in this one), "
                              "profiles/tlsfold_serve_hbm_traffic.md")
 
 
