@@ -568,9 +568,9 @@ GTLS_NQ = 4
 # FP64 operations of one (epoch, W, L) trial of the fast gtls kernel from the
 # tile prefix sums: region sums (prefix differences, the B/D moments moved to
 # epoch coordinates) 33, region-A index quirk 5, centred sums 17, denA 16,
-# numB 11, H and its halves 4, the expanded residual 37, S/N 4; +, -, *, /,
-# sqrt one each
-GTLS_TRIAL_FLOPS = 127
+# numB 11, H, W/4 and L^2 4, the expanded residual times L^2 36, the rank
+# comparison with the best trial 6; +, -, *, / one each
+GTLS_TRIAL_FLOPS = 128
 
 
 def bench_gtls(steps: int, warmup: int):

@@ -287,6 +287,19 @@ __device__ __forceinline__ void gf_first4(const GfLds &L, int e, int nsc, int to
   for (int i = 0; i < 4; ++i) out[i] = lo[i];
 }
 
+// The first row at or after which every row passes (fl32(K - deltat*k) > thr,
+// a row at or past nsc passing), from a starting row b: the test is monotone
+// in the row (K is non-decreasing), so walking down while the row below
+// passes and then up while the row fails ends there from any start.  A
+// region boundary moves by a few rows from one L to the next, so this is a
+// short walk where the search takes log2(nsc) steps.
+__device__ __forceinline__ int gf_walk(const GfLds &L, int e, int nsc, float mdk, float kf,
+                                       float thr, int b) {
+  while (b > 0 && fmaf(mdk, kf, L.K[e + b - 1]) > thr) --b;
+  while (b < nsc && !(fmaf(mdk, kf, L.K[e + b]) > thr)) ++b;
+  return b;
+}
+
 struct GfSum {  // sums over the valid rows of a union row range, tau coordinates
   double n, x, x2, t, t2, xt;
 };
@@ -403,7 +416,7 @@ __device__ void gf_scan64(const GfLds &L, int e, int nsc, int nl, float W, float
 
 // grid: (ceil(nt/E), nq); block kGfThreads; E epochs per block (E*nw <=
 // kGfThreads*kGfUnits); dynamic LDS sized by gf_lds_bytes(E + nsc - 1).
-__global__ __launch_bounds__(kGfThreads) void gtls_fast_kernel(
+__global__ __launch_bounds__(kGfThreads, 4) void gtls_fast_kernel(
     float *__restrict__ tlssn, float *__restrict__ tlsw, float *__restrict__ tlst0,
     float *__restrict__ tlsl, float *__restrict__ tlshmax, const float *__restrict__ lc,
     const float *__restrict__ tu, int nt, int nq, int nw, int nl, int nsc, float wmax,
@@ -566,6 +579,7 @@ __global__ __launch_bounds__(kGfThreads) void gtls_fast_kernel(
     if (bad) {
       gf_scan64(L, e, nsc, nl, W, deltat, kf, mode, best, bL, bH);
     } else {
+      double bh2 = 0.0, bres = 1.0;  // hv^2 (n-3) L^2 and res L^2 of the best trial
       const float dl = mode ? (float)((double)W / 4.0 / (double)(float)nl)
                             : (float)((double)W * 0.45 / (double)(float)nl);
       const float tEnd = (float)(0.5 * (double)W), tStart = (float)(-0.5 * (double)W);
@@ -582,6 +596,7 @@ __global__ __launch_bounds__(kGfThreads) void gtls_fast_kernel(
         const double dW = (double)W;
         // y = x' - mean' on the window: sum y^2 (the residual's constant)
         const double Y2 = Wn.x2 - 2.0 * meanp * Wn.x + n * meanp * meanp;
+        int bnd[4] = {0, 0, 0, 0};
         for (int mm = 0; mm < nl; ++mm) {
           const float Lf = (mode ? (float)(mm + 1) : (float)mm) * dl;
           const double dL = (double)Lf;
@@ -589,8 +604,12 @@ __global__ __launch_bounds__(kGfThreads) void gtls_fast_kernel(
           const float tBC = (float)((double)(-W) * 0.25 + (double)Lf * 0.5);
           const float tCD = -tBC, tDE = -tAB;
           const float thr[4] = {tAB, tBC, tCD, tDE};
-          int bnd[4];
-          gf_first4(L, e, nsc, top, mdk, kf, thr, bnd);
+          if (mm == 0) {
+            gf_first4(L, e, nsc, top, mdk, kf, thr, bnd);
+          } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) bnd[i] = gf_walk(L, e, nsc, mdk, kf, thr[i], bnd[i]);
+          }
           const int eA = bnd[0], eB = bnd[1], eC = bnd[2], eD = bnd[3];
           const GfSum A = gf_sum(L, e + nSx, e + eA, sft);
           const GfSum B = gf_sum(L, e + eA, e + eB, sft);
@@ -623,18 +642,29 @@ __global__ __launch_bounds__(kGfThreads) void gtls_fast_kernel(
                               8.0 * dW * (B.t - D.t) + 16.0 * (B.t2 + D.t2);
           const double numB = 8.0 * dL * dL * xAEC - 4.0 * dW * dL * xBD +
                               16.0 * dL * (ytD - ytB);
-          const double H = -numB / denA, Hh = H / 2.0, HL = H / dL, q4 = dW / 4.0;
-          const double res =
-              Y2 + 2.0 * Hh * (yA - yC + yE) + Hh * Hh * (A.n + Cc.n + Ee.n) -
-              2.0 * HL * (ytB + q4 * yB) + HL * HL * (B.t2 + 2.0 * q4 * B.t + q4 * q4 * B.n) +
-              2.0 * HL * (ytD - q4 * yD) + HL * HL * (D.t2 - 2.0 * q4 * D.t + q4 * q4 * D.n);
-          if (res > 0.0) {
-            const double hv = mode ? fabs(H) : H;
-            const double sn = hv / sqrt(res / (n - 3.0));
-            if (sn > best) { best = sn; bL = Lf; bH = H; }
+          // the residual with h = H/2 and a = H/L,
+          //   res = Y2 + 2h(yA - yC + yE) + h^2 (nA + nC + nE)
+          //         - 2a(ytB + q4 yB) + a^2 (B.t2 + 2 q4 B.t + q4^2 B.n)
+          //         + 2a(ytD - q4 yD) + a^2 (D.t2 - 2 q4 D.t + q4^2 D.n),
+          // is formed times L^2 (no division by L; L = 0 empties B and D, so
+          // that trial's scaled residual is 0, no trial, as its residual is
+          // NaN); S/N = hv / sqrt(res/(n-3)) > 0 is compared with the best's
+          // as hv^2 (n-3) L^2 / (res L^2), cross-multiplied (both positive),
+          // and formed for the best trial only
+          const double H = -numB / denA, q4 = dW / 4.0, L2 = dL * dL;
+          const double rs =
+              L2 * (Y2 + H * (yA - yC + yE) + 0.25 * H * H * (A.n + Cc.n + Ee.n)) +
+              2.0 * H * dL * ((ytD - q4 * yD) - (ytB + q4 * yB)) +
+              H * H * ((B.t2 + 2.0 * q4 * B.t + q4 * q4 * B.n) +
+                       (D.t2 - 2.0 * q4 * D.t + q4 * q4 * D.n));
+          const double hv = mode ? fabs(H) : H;
+          if (rs > 0.0 && hv > 0.0) {
+            const double h2 = hv * hv * (n - 3.0) * L2;
+            if (h2 * bres > bh2 * rs) { bh2 = h2; bres = rs; bL = Lf; bH = H; }
           }
         }
       }
+      if (bh2 > 0.0) best = sqrt(bh2 / bres);
     }
     u_sn[u] = best; u_H[u] = bH; u_L[u] = bL; u_id[u] = id;
   }
