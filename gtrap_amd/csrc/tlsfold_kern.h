@@ -757,6 +757,8 @@ __device__ __forceinline__ double tf_stat_px(const TfLds &L, int p0, int pAB, in
 }
 
 static_assert(sizeof(TfLds) >= (kTfMaxBuckets + 1) * sizeof(int), "bucket counters exceed LDS");
+static_assert((size_t)kTfMaxT * kTfThreads * 6 <= (size_t)(kTfChunk + 1) * 5,
+              "chunked-sweep sums exceed the prefix area they are spilled into");
 static_assert(sizeof(TfLds) >= kTfSortChunk * (sizeof(double) + sizeof(float) + sizeof(int)),
               "sort chunk exceeds LDS");
 
