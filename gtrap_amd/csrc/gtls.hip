@@ -21,6 +21,7 @@
 // every tied thread write, :398-404).
 #include "common.h"
 #include <math.h>
+#include <type_traits>
 
 namespace gtrap {
 
@@ -303,6 +304,9 @@ __device__ __forceinline__ int gf_walk(const GfLds &L, int e, int nsc, float mdk
 struct GfSum {  // sums over the valid rows of a union row range, tau coordinates
   double n, x, x2, t, t2, xt;
 };
+// kShift = false: the epoch is the tile's reference (s = 0), and the moved
+// moments equal the raw ones exactly (q - n*0 = q), so the shift is skipped
+template <bool kShift = true>
 __device__ __forceinline__ GfSum gf_sum(const GfLds &L, int a, int b, double s) {
   const double *pa = L.P + 5 * (size_t)a, *pb = L.P + 5 * (size_t)b;
   GfSum o;
@@ -310,9 +314,15 @@ __device__ __forceinline__ GfSum gf_sum(const GfLds &L, int a, int b, double s) 
   o.x = pb[0] - pa[0];
   o.x2 = pb[1] - pa[1];
   const double q1 = pb[2] - pa[2], q2 = pb[3] - pa[3], xq = pb[4] - pa[4];
-  o.t = q1 - o.n * s;
-  o.t2 = q2 - 2.0 * s * q1 + o.n * s * s;
-  o.xt = xq - s * o.x;
+  if constexpr (kShift) {
+    o.t = q1 - o.n * s;
+    o.t2 = q2 - 2.0 * s * q1 + o.n * s * s;
+    o.xt = xq - s * o.x;
+  } else {
+    o.t = q1;
+    o.t2 = q2;
+    o.xt = xq;
+  }
   return o;
 }
 
@@ -596,6 +606,10 @@ __global__ __launch_bounds__(kGfThreads, 4) void gtls_fast_kernel(
         const double dW = (double)W;
         // y = x' - mean' on the window: sum y^2 (the residual's constant)
         const double Y2 = Wn.x2 - 2.0 * meanp * Wn.x + n * meanp * meanp;
+        // the L sweep, with the moments' shift to epoch coordinates elided
+        // when the epoch is the tile's reference (one epoch per tile)
+        auto sweep = [&](auto shift) {
+        constexpr bool kSh = decltype(shift)::value;
         int bnd[4] = {0, 0, 0, 0};
         for (int mm = 0; mm < nl; ++mm) {
           const float Lf = (mode ? (float)(mm + 1) : (float)mm) * dl;
@@ -611,11 +625,11 @@ __global__ __launch_bounds__(kGfThreads, 4) void gtls_fast_kernel(
             for (int i = 0; i < 4; ++i) bnd[i] = gf_walk(L, e, nsc, mdk, kf, thr[i], bnd[i]);
           }
           const int eA = bnd[0], eB = bnd[1], eC = bnd[2], eD = bnd[3];
-          const GfSum A = gf_sum(L, e + nSx, e + eA, sft);
-          const GfSum B = gf_sum(L, e + eA, e + eB, sft);
-          const GfSum Cc = gf_sum(L, e + eB, e + eC, sft);
-          const GfSum D = gf_sum(L, e + eC, e + eD, sft);
-          const GfSum Ee = gf_sum(L, e + eD, e + eE, sft);
+          const GfSum A = gf_sum<kSh>(L, e + nSx, e + eA, sft);
+          const GfSum B = gf_sum<kSh>(L, e + eA, e + eB, sft);
+          const GfSum Cc = gf_sum<kSh>(L, e + eB, e + eC, sft);
+          const GfSum D = gf_sum<kSh>(L, e + eC, e + eD, sft);
+          const GfSum Ee = gf_sum<kSh>(L, e + eD, e + eE, sft);
           // region A's flux sum with the index quirk: the flux of scoop rows
           // [0, eA - nSx) for the time tests of rows [nSx, eA)
           double sAq;
@@ -663,6 +677,8 @@ __global__ __launch_bounds__(kGfThreads, 4) void gtls_fast_kernel(
             if (h2 * bres > bh2 * rs) { bh2 = h2; bres = rs; bL = Lf; bH = H; }
           }
         }
+        };
+        if (sft == 0.0) sweep(std::false_type{}); else sweep(std::true_type{});
       }
       if (bh2 > 0.0) best = sqrt(bh2 / bres);
     }
