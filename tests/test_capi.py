@@ -35,3 +35,25 @@ def test_error_reporting_without_gpu():
     assert st == -1
     assert b"null" in lib.gtrap_last_error()
     assert lib.gtrap_version().startswith(b"gtrap_amd")
+
+
+def test_host_validation_of_tlsfold_settings_and_sizes():
+    """Launch-geometry settings and workspace sizes are checked and computed on
+    the host, before any HIP call: out-of-range tile settings are refused with
+    the allowed range in the message, the valid ones are accepted (ending on
+    the built-in geometry), and the workspace grows with nmax and nblocks."""
+    lib = _native.load()
+    for bad in ((0, 1, 2, 0), (145, 1, 2, 0), (144, 3, 2, 0), (144, 1, 9, 0), (144, 1, 2, 49)):
+        assert lib.gtrap_tlsfold_set_tiles(*bad) == -1, bad
+        assert b"gtrap_tlsfold_set_tiles" in lib.gtrap_last_error()
+    assert lib.gtrap_tlsfold_set_tiles(96, 2, 1, 24) == 0
+    assert lib.gtrap_tlsfold_set_tiles(144, 2, 2, 0) == 0
+    one = lib.gtrap_tlsfold_ws_bytes(1000, 1)
+    assert one >= 256 + 1000 * 20 and (one - 256) % 256 == 0
+    assert lib.gtrap_tlsfold_ws_bytes(1000, 8) == 256 + 8 * (one - 256)
+    assert lib.gtrap_tlsfold_ws_bytes(70000, 1) > one
+    # geometry arguments of the static entry point are refused before any
+    # device work (no current device needed for that)
+    st = lib.gtrap_tlsfold_static(None, None, None, None, None, None, None, None, None, 0, None,
+                                  None, 1, 1, None, 10, 10, 0.0204, 0, 64, None)
+    assert st == -1
