@@ -33,7 +33,7 @@ TRIAL_FLOPS = 103
 # (factors 2.0 and 1.0)
 SERVE_FOLD_TRAFFIC = 1.212e11
 SERVE_FOLD_TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this workload "
-                             "on this build (separate runs, not measured in this one), "
+                             "of the same library (separate runs, not measured in this one), "
                              "profiles/tlsfold_serve_hbm_traffic.md")
 
 
@@ -506,7 +506,7 @@ def bench_precision(steps: int, warmup: int):
     lds_peak = LDS_PEAK_TBS
     return dict(
         metric="precision study: SR / SDE error vs float64 (geebls arithmetic variants)",
-        value=own["sde_rel_max"], unit="max relative SDE error (this build's arithmetic)",
+        value=own["sde_rel_max"], unit="max relative SDE error (gtrap_geebls arithmetic)",
         ms_per_step=wall * 1e3, higher_is_better=False, dtype="f32",
         variants=res,
         tls_variants=dict(results=tls_res, workload="tls_fold on 32 of the curves (262144 "

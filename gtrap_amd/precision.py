@@ -8,7 +8,7 @@ storage     ``f32`` (lc - offset in float32) or ``f16`` (binary16 after
             (lc - offset) * 2^k per curve)
 accum       ``atomic32`` (float32 LDS atomics + float32 running sum, the
             reference's arithmetic), ``exact32`` (exact bin sums rounded to
-            float32 + float32 running sum, this build's geebls), ``twosum32``
+            float32 + float32 running sum, gtrap_geebls), ``twosum32``
             (exact bins + compensated float32 running sum), ``f64``
 fold        ``f32`` (frac(tu*f) in float32 from float32 time, the reference's)
             or ``f64`` (float64 time and frequency)
@@ -28,7 +28,7 @@ ACC = {"atomic32": 0, "exact32": 1, "twosum32": 2, "f64": 3}
 TRUTH = ("f32", "f64", "f64")
 
 # (storage, accumulation, fold): one axis changed from the float64 yardstick,
-# then the reference's arithmetic and this build's default
+# then the reference's arithmetic and gtrap_geebls's
 STUDY = (
     ("f32", "atomic32", "f64"), ("f32", "exact32", "f64"), ("f32", "twosum32", "f64"),
     ("f32", "f64", "f32"), ("f16", "f64", "f64"),

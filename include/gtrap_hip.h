@@ -27,7 +27,7 @@ extern "C" {
 #define GTRAP_EINVAL         -1  /* invalid argument (shape, size, constraint)  */
 #define GTRAP_ELAUNCH        -2  /* kernel launch / HIP runtime error             */
 #define GTRAP_ENOMEM         -3  /* workspace too small / allocation failure      */
-#define GTRAP_EUNSUPPORTED   -4  /* valid request outside what this build serves  */
+#define GTRAP_EUNSUPPORTED   -4  /* valid request outside what the library serves */
 
 /* Human-readable text of the last error on the calling thread. */
 const char *gtrap_last_error(void);
