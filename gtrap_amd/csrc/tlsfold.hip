@@ -36,12 +36,12 @@ TfTiles g_tiles;
 }  // namespace gtrap
 
 // Workspace: a 256-byte header (work ticket) plus one slot per workgroup with
-// the phase-sorted curve (16-byte records: float64 phase, float32 flux) and the bucket offsets.
+// the phase-sorted curve (16-byte records: float64 phase, float32 flux), the
+// bucket offsets and the boundary rows of a wide pass (tf_block_bytes; every
+// block's slot 256-byte aligned).
 extern "C" int64_t gtrap_tlsfold_ws_bytes(int32_t nmax, int32_t nblocks) {
   if (nmax < 1) nmax = 1;
-  int64_t per = (int64_t)nmax * 16 + (int64_t)(nmax + 2) * 4;
-  per = (per + 255) / 256 * 256;  // keep every block's slot 256-byte aligned
-  return 256 + per * (int64_t)(nblocks > 0 ? nblocks : 1);
+  return 256 + gtrap::tf_block_bytes(nmax) * (int64_t)(nblocks > 0 ? nblocks : 1);
 }
 
 static int64_t tf_slot_bytes(int32_t nmax) { return (gtrap_tlsfold_ws_bytes(nmax, 1) - 256); }

@@ -52,6 +52,15 @@ __device__ __forceinline__ double tf_fold(float t, double P, double invP) {
   }
 }
 
+// A value every lane of the wave holds (read from lane 0): the compiler then
+// keeps it in a scalar register and branches on it with scalar branches.
+__device__ __forceinline__ int tf_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int64_t tf_uniform64(int64_t v) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(v >> 32));
+  return (int64_t)(((unsigned long long)hi << 32) | lo);
+}
+
 // Total order on float bit patterns (signed-magnitude to two's complement);
 // breaks ties between samples of equal phase deterministically, NaN included.
 __device__ __forceinline__ int tf_xkey(float x) {
@@ -181,13 +190,6 @@ __device__ __forceinline__ int tf_bucket_bound(const int *cursor, int nb, int nv
   return idx < -nv ? -nv : (idx > 2 * nv ? 2 * nv : idx);
 }
 
-struct TfSums {  // sums over a sample range in epoch coordinates
-  double n, s1, s2, d1, d2, xd;
-};
-
-// The statistic of one trial from cumulative sums at its boundaries
-// (cumulative from the window start): c0 = -W/2, cAB..cDE, cE = W/2 (<),
-// cM = W/2 (<=).  Returns S/N; writes H.
 // Region sums of one trial: counts n*, flux sums s* (x' = x - x_ref), and for
 // the ingress/egress regions B and D the sums of d, d^2 and x'd (d = offset
 // from the epoch); the window's mean of x' over [-W/2, W/2] and its sum of
@@ -277,26 +279,6 @@ __device__ __forceinline__ double tf_sn(double key) {
 }
 __device__ __forceinline__ double tf_h(double numB, double denA) { return -numB / denA; }
 
-// The statistic from cumulative sums at the trial's boundaries (cumulative
-// from the window start, epoch coordinates): c0 = -W/2, cAB..cDE, cE = W/2
-// (<), cM = W/2 (<=).
-__device__ __forceinline__ double tf_stat(const TfSums &c0, const TfSums &cAB,
-                                          const TfSums &cBC, const TfSums &cCD,
-                                          const TfSums &cDE, const TfSums &cE,
-                                          const TfSums &cM, double W, const double *c, int stat,
-                                          double *numB, double *denA) {
-  TfRegions g;
-  tf_window(cM.n - c0.n, cM.s1 - c0.s1, cE.n - c0.n, cE.s1 - c0.s1, cE.s2 - c0.s2, g.mean, g.y2);
-  g.nA = cAB.n - c0.n; g.nB = cBC.n - cAB.n; g.nC = cCD.n - cBC.n;
-  g.nD = cDE.n - cCD.n; g.nE = cE.n - cDE.n;
-  g.sA = cAB.s1 - c0.s1; g.sB = cBC.s1 - cAB.s1; g.sC = cCD.s1 - cBC.s1;
-  g.sD = cDE.s1 - cCD.s1; g.sE = cE.s1 - cDE.s1;
-  g.dB = cBC.d1 - cAB.d1; g.dD = cDE.d1 - cCD.d1;
-  g.d2B = cBC.d2 - cAB.d2; g.d2D = cDE.d2 - cCD.d2;
-  g.xdB = cBC.xd - cAB.xd; g.xdD = cDE.xd - cCD.xd;
-  return tf_eval(g, W, c, stat, numB, denA);
-}
-
 struct TfArgs {
   const float *t, *x;
   const int64_t *off;
@@ -339,7 +321,6 @@ constexpr int kTfWl = 10;  // (W, L) table row: tAB, tBC, tCD, tDE, 4L^2, 8L^2, 
 constexpr int kTfCells = 2 * kTfChunk;  // phase index cells of a staged range (at most)
 constexpr int kTfMaxTile = 144;  // epochs per staged tile: with the reference at the
                                  // tile centre, |q| stays within 72 epoch steps + W/2
-constexpr int kTfMaxT = 4;    // chunked sweep: (epoch, W, boundary) sums per thread
 constexpr int kTfMaxBuckets = 30000;  // phase buckets (LDS counters, <= 120 KB)
 constexpr int kTfSortChunk = 7680;    // samples per in-LDS bucket-sort chunk (90 KB + ends)
 constexpr int kTfMaxBreaks = 128;     // sort chunks per period (longer curves: one bitonic sort)
@@ -623,21 +604,6 @@ __device__ __forceinline__ const double *tf_px(const TfLds &L, int p) {
   return &L.px[0][0] + __umul24((unsigned)p, 5u);
 }
 
-// cumulative sums (epoch coordinates, shift s = t0 - r) at staged index p
-__device__ __forceinline__ TfSums tf_cum(const TfLds &L, int p, double s) {
-  const double n = (double)p;
-  const double *row = tf_px(L, p);
-  const double x1 = row[0], x2 = row[1], q1 = row[2], q2 = row[3], xq = row[4];
-  TfSums o;
-  o.n = n;
-  o.s1 = x1;
-  o.s2 = x2;
-  o.d1 = q1 - n * s;
-  o.d2 = q2 - 2.0 * s * q1 + n * s * s;
-  o.xd = xq - s * x1;
-  return o;
-}
-
 // tf_search over the staged range through the phase index, f = (t0 + c -
 // tbase) * tinv the target's fractional cell.  Cells are at least 1e-9 of the
 // magnitudes wide (tf_stage), so the rounding of f and of the offset test is
@@ -731,13 +697,14 @@ __device__ __forceinline__ int tf_bisect(const TfLds &L, int a, int b, double t0
 // The statistic from the staged prefix sums at the trial's boundary indices:
 // raw differences first (x', x'^2 for the window and regions; q, q^2, x'q
 // only for B and D), shifted to epoch coordinates (d = q - s) per region.
-__device__ __forceinline__ double tf_stat_px(const TfLds &L, int p0, int pAB, int pBC, int pCD,
-                                             int pDE, int pE, double mean, double y2, double s,
-                                             double W, const double *c, int stat, double *numB,
-                                             double *denA) {
+// r0..rE are the prefix rows (x', x'^2, q, q^2, x'q) at the boundary indices.
+__device__ __forceinline__ double tf_stat_rows(int p0, int pAB, int pBC, int pCD, int pDE,
+                                               int pE, const double *r0, const double *rAB,
+                                               const double *rBC, const double *rCD,
+                                               const double *rDE, const double *rE, double mean,
+                                               double y2, double s, double W, const double *c,
+                                               int stat, double *numB, double *denA) {
 #pragma clang fp contract(fast)
-  const double *r0 = tf_px(L, p0), *rAB = tf_px(L, pAB), *rBC = tf_px(L, pBC),
-               *rCD = tf_px(L, pCD), *rDE = tf_px(L, pDE), *rE = tf_px(L, pE);
   const double x0 = r0[0], xAB = rAB[0], xBC = rBC[0], xCD = rCD[0], xDE = rDE[0], xE = rE[0];
   TfRegions g;
   g.mean = mean;
@@ -755,10 +722,45 @@ __device__ __forceinline__ double tf_stat_px(const TfLds &L, int p0, int pAB, in
   g.xdD = xqD - s * g.sD;
   return tf_eval(g, W, c, stat, numB, denA);
 }
+__device__ __forceinline__ double tf_stat_px(const TfLds &L, int p0, int pAB, int pBC, int pCD,
+                                             int pDE, int pE, double mean, double y2, double s,
+                                             double W, const double *c, int stat, double *numB,
+                                             double *denA) {
+  return tf_stat_rows(p0, pAB, pBC, pCD, pDE, pE, tf_px(L, p0), tf_px(L, pAB), tf_px(L, pBC),
+                      tf_px(L, pCD), tf_px(L, pDE), tf_px(L, pE), mean, y2, s, W, c, stat, numB,
+                      denA);
+}
+
+// One boundary of a wide pass, resolved in the workspace: the prefix row
+// (x', x'^2, q, q^2, x'q; cumulative from the pass's first sample, q relative
+// to the pass's reference) at the boundary's index, and that index counted
+// from the pass's first sample.
+struct __align__(16) TfSlot {
+  double row[5];
+  int idx, pad;
+};
+// (epoch, W, boundary) slots of one wide pass: nmax/4 within [1024, 16384]
+// (at least the 16 x 47 boundaries of one epoch at the largest nw and nl; a
+// wide pass needs a curve of several thousand samples)
+constexpr int kTfPassSlotsMin = 1024, kTfPassSlotsMax = 16384;
+__host__ __device__ inline int tf_pass_slots(int64_t nmax) {
+  const int64_t s = nmax / 4;
+  return (int)(s < kTfPassSlotsMin ? kTfPassSlotsMin : (s > kTfPassSlotsMax ? kTfPassSlotsMax : s));
+}
+
+// Workspace slot of one workgroup: the phase-sorted records (nmax of 16
+// bytes), the bucket ends (nmax + 2 ints), then, 16-byte aligned, the
+// tf_pass_slots(nmax) slots of a wide pass.
+__host__ __device__ inline int64_t tf_pass_off(int64_t nmax) {
+  return (nmax * 16 + (nmax + 2) * 4 + 15) / 16 * 16;
+}
+__host__ __device__ inline int64_t tf_block_bytes(int64_t nmax) {
+  return (tf_pass_off(nmax) + (int64_t)tf_pass_slots(nmax) * (int64_t)sizeof(TfSlot) + 255) /
+         256 * 256;
+}
 
 static_assert(sizeof(TfLds) >= (kTfMaxBuckets + 1) * sizeof(int), "bucket counters exceed LDS");
-static_assert((size_t)kTfMaxT * kTfThreads * 6 <= (size_t)(kTfChunk + 1) * 5,
-              "chunked-sweep sums exceed the prefix area they are spilled into");
+static_assert(kTfPassSlotsMin >= kTfMaxNw * kTfMaxBnd, "a wide pass must hold one epoch");
 static_assert(sizeof(TfLds) >= kTfSortChunk * (sizeof(double) + sizeof(float) + sizeof(int)),
               "sort chunk exceeds LDS");
 
@@ -807,6 +809,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
   unsigned char *ws = A.ws + 256 + (size_t)blockIdx.x * (size_t)A.ws_per_block;
   TfRec *rec = reinterpret_cast<TfRec *>(ws);
   int *cursor = reinterpret_cast<int *>(rec + A.nmax);
+  TfSlot *slots = reinterpret_cast<TfSlot *>(ws + tf_pass_off(A.nmax));
   int *ticket = A.counter ? A.counter : reinterpret_cast<int *>(A.ws);
   double wmax = 0.0;
   for (int w = 0; w < nw; ++w) wmax = fmax(wmax, A.widths[w]);
@@ -818,14 +821,18 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
     // ---- next work item (atomic ticket; packed batches run longest period first)
     if (tid == 0) s_item = (int64_t)atomicAdd(ticket, 1);
     __syncthreads();
-    const int64_t slot = s_item;
+    // The item's scalars are the same in every lane; read them as such
+    // (readfirstlane), so that every branch on them is a scalar branch (the
+    // compiler cannot tell on its own that values loaded through the LDS
+    // ticket are uniform).
+    const int64_t slot = tf_uniform64(s_item);
     __syncthreads();
     if (slot >= nitems) break;
     int c;
     int64_t item;
     if (kItems) {
-      c = A.item_curve[slot];
-      item = A.item_pidx[slot];
+      c = tf_uniform(A.item_curve[slot]);
+      item = tf_uniform64(A.item_pidx[slot]);
     } else {
       item = A.nptot - 1 - slot;
       int lo_c = 0, hi_c = A.ncurves - 1;
@@ -848,7 +855,7 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
         return xc[i];
       }
     };
-    const int n = A.len[c];
+    const int n = tf_uniform(A.len[c]);
     const int nE = (int)ceil(P / A.dtep);
     if (n > A.nmax) {
       // a curve longer than the workspace slot is not searched (the host
@@ -861,6 +868,12 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
         A.bl[item] = -2;
         A.be[item] = -2;
       }
+      // every lane meets here before the next ticket, as at the end of a
+      // searched item: without this barrier the compiler joined thread 0's
+      // writes to the next ticket's atomic (both under tid == 0), so thread 0
+      // and the other lanes of its wave reached the loop's first barrier by
+      // different paths and the wave never reconverged
+      __syncthreads();
       continue;
     }
 
@@ -1157,78 +1170,88 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
           __syncthreads();
         }
       } else {
-        // ---- chunked sweep: a window larger than the LDS tile.  Up to
-        // kTfMaxT * kTfThreads (epoch, W, boundary) partial sums over a run
-        // of epochs are carried in registers across the chunks, then spilled
-        // into the (idle) prefix area for the trial pass.
-        const int tce = max(1, min(nE - e0, (kTfMaxT * kTfThreads) / (nw * nbnd)));
-        const double r = (double)e0 * A.dtep;
+        // ---- wide pass: a window larger than the LDS tile (dense items).
+        // The union of the windows of up to tf_pass_slots / (nw nbnd) epochs
+        // is staged chunk by chunk, the prefix sums carried from chunk to
+        // chunk (one reference q = image - r for the whole pass).  Every
+        // (epoch, W, boundary) is resolved once, in the chunk holding its
+        // first passing sample, and its prefix row goes to the workspace;
+        // the trials then read their seven rows from there.
+        const int nkind = nw * nbnd;
+        // epochs per pass from the curve's own length (n <= nmax: the
+        // workspace holds them), so an item's rounding is the same in any batch
+        const int tce = max(1, min(nE - e0, tf_pass_slots(n) / nkind));
+        const double r = (double)(e0 + (tce - 1) / 2) * A.dtep;  // central epoch
         const int chi = tf_bucket_bound(cursor, nb, nv, P, bscale,
                                         (double)(e0 + tce - 1) * A.dtep + hwm + marg, 1);
-        const int ntrip = tce * nw * nbnd;
-        double acc[kTfMaxT][6];
-        // each slot's (epoch, boundary), the same for every chunk of the pass
-        double jt0[kTfMaxT], jc[kTfMaxT];
-        int jkind[kTfMaxT];
-#pragma unroll
-        for (int j = 0; j < kTfMaxT; ++j) {
-#pragma unroll
-          for (int v = 0; v < 6; ++v) acc[j][v] = 0.0;
-          const int k = tid + j * kTfThreads;
-          jt0[j] = 0.0;
-          jc[j] = 0.0;
-          jkind[j] = 0;
-          if (k < ntrip) {
-            const int el = k / (nw * nbnd), rem = k % (nw * nbnd);
-            const int w = rem / nbnd, b = rem % nbnd;
-            jt0[j] = (double)(e0 + el) * A.dtep;
-            tf_bnd(sw[w], wlt + kTfWl * w * nl, b, jc[j], jkind[j]);
-          }
-        }
+        double carry[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        double prev = -INFINITY;  // image of the previous chunk's last sample
         for (int ca = lo; ca < chi; ca += kTfChunk) {
           const int cc = min(kTfChunk, chi - ca);
+          const bool first = ca == lo, last = ca + cc >= chi;
           tf_stage<kVar>(L, rec, nv, P, ca, cc, r, xref, A.cells);
-#pragma unroll
-          for (int j = 0; j < kTfMaxT; ++j) {
-            const int k = tid + j * kTfThreads;
-            if (k < ntrip) {
-              const double t0 = jt0[j], cval = jc[j];
-              int a, b;
-              tf_bracket(L, t0, cval, a, b);
-              int p = tf_bisect(L, a, b, t0, cval, jkind[j]);
-              if (b - a > kTfCount) p = tf_search(L.lu, a, b, t0, cval, jkind[j]);
+          const double ptop = L.lu[cc - 1];
+          for (int kd = tid; kd < nkind; kd += kTfThreads) {
+            const int w = kd / nbnd, b = kd - w * nbnd;
+            double c;
+            int kind;
+            tf_bnd(sw[w], wlt + kTfWl * w * nl, b, c, kind);
+            // candidate epochs: boundary value t0 + c in [prev, ptop], with
+            // a margin of 1e-6 epoch steps each side (the exact test decides)
+            int ea = e0, eb = e0 + tce - 1;
+            if (!first) ea = max(ea, (int)ceil((prev - c) / A.dtep - 1e-6));
+            if (!last) eb = min(eb, (int)floor((ptop - c) / A.dtep + 1e-6));
+            for (int e = ea; e <= eb; ++e) {
+              const double t0 = (double)e * A.dtep;
+              int a, bb;
+              tf_bracket(L, t0, c, a, bb);
+              int p = tf_bisect(L, a, bb, t0, c, kind);
+              if (bb - a > kTfCount) p = tf_search(L.lu, a, bb, t0, c, kind);
               p = tf_clamp(p, 0, cc);
-              const TfSums cs = tf_cum(L, p, t0 - r);
-              acc[j][0] += cs.n; acc[j][1] += cs.s1; acc[j][2] += cs.s2;
-              acc[j][3] += cs.d1; acc[j][4] += cs.d2; acc[j][5] += cs.xd;
+              // the boundary is in this chunk when its first passing sample
+              // is: inside the chunk (or past the last chunk's end), and at
+              // the chunk's start only if the previous chunk's last sample
+              // fails the test
+              bool take = p < cc || last;
+              if (p == 0 && !first) {
+                const double d = prev - t0;
+                take = !(kind ? d > c : d >= c);
+              }
+              if (take) {
+                const double *row = tf_px(L, p);
+                TfSlot sl;
+#pragma unroll
+                for (int v = 0; v < 5; ++v) sl.row[v] = carry[v] + row[v];
+                sl.idx = ca - lo + p;
+                sl.pad = 0;
+                slots[(size_t)(e - e0) * nkind + kd] = sl;
+              }
             }
           }
+#pragma unroll
+          for (int v = 0; v < 5; ++v) carry[v] += L.px[cc][v];
+          prev = ptop;
           __syncthreads();
         }
-        double *accl = &L.px[0][0];  // tce*nw*nbnd*6 <= (kTfChunk+1)*5 doubles
-#pragma unroll
-        for (int j = 0; j < kTfMaxT; ++j) {
-          const int k = tid + j * kTfThreads;
-          if (k < ntrip)
-#pragma unroll
-            for (int v = 0; v < 6; ++v) accl[6 * k + v] = acc[j][v];
-        }
-        __syncthreads();
+        // one thread per (epoch, W, L >= 1) trial of the pass
         for (int k = tid; k < tce * nw * nlt; k += kTfThreads) {
           const int mm = k % nlt, jj = k / nlt, m = mm + 1;
           const int el = jj / nw, w = jj % nw;
           const double W = sw[w];
-          const double *base = accl + 6 * ((size_t)(el * nw + w) * nbnd);
-          auto get = [&](int b) {
-            const double *v = base + 6 * b;
-            return TfSums{v[0], v[1], v[2], v[3], v[4], v[5]};
-          };
-          double nB = NAN, dA = NAN;
-          const double key =
-              W < P ? tf_stat(get(0), get(3 + 4 * mm), get(4 + 4 * mm), get(5 + 4 * mm),
-                              get(6 + 4 * mm), get(1), get(2), W, wlt + kTfWl * (w * nl + m),
-                              A.stat, &nB, &dA)
-                    : __builtin_nan("");  // W >= P: no trial
+          const TfSlot *sb = slots + (size_t)el * nkind + w * nbnd;
+          const TfSlot *s0 = sb, *sE = sb + 1, *sM = sb + 2, *sAB = sb + 3 + 4 * mm,
+                       *sBC = sAB + 1, *sCD = sAB + 2, *sDE = sAB + 3;
+          double nB = NAN, dA = NAN, key = __builtin_nan("");
+          if (W < P) {  // W >= P: no trial
+            double mean, y2;
+            tf_window((double)(sM->idx - s0->idx), sM->row[0] - s0->row[0],
+                      (double)(sE->idx - s0->idx), sE->row[0] - s0->row[0],
+                      sE->row[1] - s0->row[1], mean, y2);
+            const double t0 = (double)(e0 + el) * A.dtep;
+            key = tf_stat_rows(s0->idx, sAB->idx, sBC->idx, sCD->idx, sDE->idx, sE->idx, s0->row,
+                               sAB->row, sBC->row, sCD->row, sDE->row, sE->row, mean, y2, t0 - r,
+                               W, wlt + kTfWl * (w * nl + m), A.stat, &nB, &dA);
+          }
           const unsigned long long idx = tf_idx(w, m, e0 + el, nl);
           if (tf_better(key, idx, best)) best = TfBest{key, nB, dA, idx};
         }
