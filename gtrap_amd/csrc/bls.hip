@@ -27,7 +27,15 @@
 
 namespace gtrap {
 
-constexpr int kBlsThreads = 256;
+// 512 threads and at most 52 KB of bins per workgroup: three workgroups (24
+// waves) per CU keep the LDS atomics' latency covered.  On 1024 curves x
+// 16384 samples x 2048 frequencies (nb 1024, kma 22): 256 threads and 64 KB
+// (two workgroups of 4 waves) 72.4 ms, 512 threads and 64 KB 45.0 ms, 1024
+// threads and 64 KB 40.8 ms, 512 threads and 40 KB 38.7 ms, 512 and 52 KB
+// 35.3 ms.  Bin sums are order-independent, so the outputs do not depend on
+// either choice.
+constexpr int kBlsThreads = 512;
+constexpr size_t kBlsLdsBudget = 52 * 1024;
 constexpr int kBlsSpt = 16;  // samples per thread per register tile
 
 __device__ __forceinline__ int bls_scale_exp(float maxabs, uint32_t n) {
@@ -281,7 +289,7 @@ static int bls_launch(bool simple, float *blsp, uint32_t *phase1, uint32_t *phas
   }
   const uint32_t nbk = nb + kma;
   const size_t per_f = (size_t)nbk * 12;
-  int fpb = (int)((64 * 1024 - 4 * (size_t)nbk) / per_f);
+  int fpb = 4 * (size_t)nbk < kBlsLdsBudget ? (int)((kBlsLdsBudget - 4 * (size_t)nbk) / per_f) : 1;
   if (fpb > 8) fpb = 8;
   if (fpb < 1) fpb = 1;
   GTRAP_REQUIRE(per_f * fpb + 4 * (size_t)nbk <= 150 * 1024,
