@@ -22,15 +22,25 @@
 namespace gtrap {
 
 // Delete `ov` (known to be present) from the sorted array s and insert `nv`.
+// d[k] = s[k] < ov ? s[k] : s[k+1] is the array with ov deleted (+inf at the
+// end); the new s[k] = med3(d[k-1], nv, d[k]) (s[0] = min(nv, d[0])) inserts
+// nv.  Slot k is rewritten as soon as d[k] and d[k+1] exist, so only two d
+// values are live at a time and the window needs about W + 4 registers (the
+// whole d array kept 2W + a few live: one wave per SIMD at W = 129).
 template <int W>
 __device__ __forceinline__ void slide_sorted(float (&s)[W], float ov, float nv) {
-  float d[W];
+  static_assert(W > 1, "a window of at least two slots");
+  float dprev = (s[0] < ov) ? s[0] : s[1];
+  float snew = fminf(nv, dprev);  // the new s[0]
 #pragma unroll
-  for (int k = 0; k < W - 1; ++k) d[k] = (s[k] < ov) ? s[k] : s[k + 1];
-  d[W - 1] = (s[W - 1] < ov) ? s[W - 1] : INFINITY;
-  s[0] = fminf(nv, d[0]);
-#pragma unroll
-  for (int k = 1; k < W; ++k) s[k] = med3(d[k - 1], nv, d[k]);
+  for (int k = 1; k < W; ++k) {
+    const float nxt = (k + 1 < W) ? s[k + 1 < W ? k + 1 : k] : INFINITY;
+    const float dcur = (s[k] < ov) ? s[k] : nxt;
+    s[k - 1] = snew;  // old s[k-1] was last read by d[k-1]
+    snew = med3(dprev, nv, dcur);
+    dprev = dcur;
+  }
+  s[W - 1] = snew;
 }
 
 constexpr int kUnroll = 8;
