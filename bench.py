@@ -27,14 +27,18 @@ FP64_PEAK = 78.6  # TFLOP/s, MI355X dense FP64
 # residual and rank key sign(S/N)*(S/N)^2 (74); +, -, *, / count one each,
 # a fused multiply-add two
 TRIAL_FLOPS = 103
-# HBM bytes per serving-step fold launch of the default serve workload,
-# measured offline: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch of
-# tlsfold_kernel<true>, scaled by the same counters over a 1 GiB copy
-# (factors 2.0 and 1.0)
-SERVE_FOLD_TRAFFIC = 1.212e11
+# HBM bytes of the serving steps' fold per evaluated trial, measured offline:
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch of
+# tlsfold_kernel<true> over the default serve workload, scaled by the same
+# counters over a 1 GiB copy (factors 2.0 and 1.0): 13.69 B read + 14.73 B
+# written per evaluated trial.  The line's traffic is this times the run's
+# evaluated trials per launch (launches split the trace differently from run
+# to run, so a per-launch constant would not carry over).
+SERVE_FOLD_BYTES_PER_TRIAL = 28.42
 SERVE_FOLD_TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this workload "
-                             "of the same library (separate runs, not measured in this one), "
-                             "profiles/tlsfold_serve_hbm_traffic.md")
+                             "of the same library (separate runs, not measured in this one): "
+                             "28.42 HBM bytes per evaluated trial times this run's evaluated "
+                             "trials per launch, profiles/tlsfold_serve_hbm_traffic.md")
 
 
 def log(msg: str) -> None:
@@ -337,7 +341,8 @@ def bench_serve(steps: int, warmup: int):
                     requests_per_step=SERVE_REQUESTS, items_per_step=cfg.items_per_step,
                     nblocks=cfg.nblocks, hbm_budget_gib=cfg.hbm_budget / 2 ** 30),
         roofline=dict(bound="fp64-valu", achieved=ach, peak=FP64_PEAK, unit="TFLOP/s",
-                      frac=ach / FP64_PEAK, traffic=SERVE_FOLD_TRAFFIC,
+                      frac=ach / FP64_PEAK,
+                      traffic=SERVE_FOLD_BYTES_PER_TRIAL * evaluated / nlaunch,
                       traffic_source=SERVE_FOLD_TRAFFIC_SOURCE,
                       kernel="tlsfold_kernel<true> (serving step)", launches=nlaunch,
                       kernel_ms=kms, trials_per_launch=ktr / max(nlaunch, 1),

@@ -125,6 +125,15 @@ def test_longest_serving_curve(gpu):
     _check([(t, x)], [np.array([0.6, 23.3, 466.0])], np.linspace(0.04, 0.3, 10), 10)
 
 
+def test_dense_periods_over_many_wide_passes(gpu):
+    """Windows wider than the LDS tile at every epoch (70000 samples folded at
+    3-8 d): each period takes 4-10 wide passes of up to 42 epochs, each pass
+    staged in several chunks, so many (epoch, W, boundary) rows are resolved
+    at a chunk's first or last sample; the full 10 W x 10 L grid."""
+    t, x = _curve(70000, 24, mask=0.02, inject=(5.0, 1.3, 0.12, 0.01, 2e-3))
+    _check([(t, x)], [np.array([3.1, 5.0, 8.2])], np.linspace(0.04, 0.3, 10), 10)
+
+
 def test_gapped_curve(gpu):
     """Two 13-day sectors with a 1-day mid-sector gap each and 10 days between
     them: at periods near and above the gap lengths whole stretches of phase
