@@ -363,7 +363,7 @@ int gtrap_tlsfold_set_tiles(int32_t max_tile_epochs, int32_t cells_per_sample,
  * slots live in a static 64 MiB device array per GPU (held from the
  * library's load until the process ends); it calls no hipMalloc/hipFree.
  * nblocks is clamped to the slots that fit: 64 MiB / gtrap_tlsfold_ws_bytes
- * (nmax, 1), e.g. 47 workgroups for nmax = 70000, 816 for nmax = 4096; an
+ * (nmax, 1), e.g. 30 workgroups for nmax = 70000, 511 for nmax = 4096; an
  * nmax whose single slot does not fit (about 3.3 million samples) is
  * rejected with GTRAP_EINVAL.  Results are identical to gtrap_tlsfold's.
  * Calls that overlap on one GPU (other streams or host threads) share the

@@ -53,7 +53,7 @@ def _call_static(b, widths, nblocks):
 def test_static_workspace_matches_caller_workspace(gpu):
     import torch
     widths = torch.from_numpy(np.linspace(0.04, 0.3, 10)).cuda()
-    # first call: long curves (the slot count clamps the 512 workgroups to 47),
+    # first call: long curves (the slot count clamps the 512 workgroups to 30),
     # second call: a smaller batch on the same static array
     for lens, nper, seed in (((70000, 9000, 2500), (40, 60, 30), 1), ((3000, 800), (25, 9), 2)):
         b = _batch(lens, nper, seed)
