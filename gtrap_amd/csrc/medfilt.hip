@@ -16,6 +16,11 @@
 // image consecutive lanes read consecutive curves: one wave reads 256
 // contiguous bytes per row (coalesced), unlike the reference's (1,1000)
 // blocks that read one curve per block with stride nq.
+//
+// Windows wider than 129 slots (r > 64, the reference drivers' r = 100, 128
+// and 512) keep one sorted window across a group of 2-16 lanes (129 slots
+// each, DPP moves between neighbours); windows beyond 16 lanes (r > 1031) use
+// an exact bisection over the float key space on LDS tiles.
 #include "common.h"
 #include <math.h>
 
@@ -140,6 +145,136 @@ __global__ __launch_bounds__(256) void medfilt_ragged_kernel(
   medfilt_run<RMAX>(x + off[c], y + off[c], 1, n, i0, i1, r, mode);
 }
 
+// ---- wide windows in registers: one sorted window over a group of G lanes ----
+//
+// Lane gp (0..G-1) of a group holds slots [gp*WL, (gp+1)*WL) of one sorted
+// array of T = G*WL slots; the median sits at slot T/2 - 1, the last slot of
+// lane G/2 - 1, so windows up to r = T/2 - 1 fit (the rest are -inf/+inf
+// sentinels, as in the one-lane window).  A slide is the one-lane slide over
+// the whole array: slot k becomes med3(d[k-1], nv, d[k]) with d[k] = s[k] if
+// s[k] < ov else s[k+1]; a lane needs only its right neighbour's first slot
+// (for its last d) and its left neighbour's last d (for its first slot), two
+// DPP moves inside a row of 16 lanes (G divides 16, groups never straddle a
+// row).  Work per output is 3 T VALU ops spread over G lanes.
+
+// lane i reads lane i+1 (row_shl:1) or lane i-1 (row_shr:1) of its row
+__device__ __forceinline__ float dpp_from_next(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x101, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float dpp_from_prev(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xf, 0xf, true));
+}
+
+template <int WL, int G>
+__device__ __forceinline__ void slide_group(float (&s)[WL], float ov, float nv, int gp) {
+  float nxt0 = dpp_from_next(s[0]);
+  if (gp == G - 1) nxt0 = INFINITY;
+  const float dlast = (s[WL - 1] < ov) ? s[WL - 1] : nxt0;
+  float dprev = dpp_from_prev(dlast);
+  if (gp == 0) dprev = -INFINITY;  // med3(-inf, nv, d[0]) = min(nv, d[0])
+#pragma unroll
+  for (int k = 0; k < WL; ++k) {
+    const float dcur = (k + 1 < WL) ? ((s[k] < ov) ? s[k] : s[k + 1 < WL ? k + 1 : k]) : dlast;
+    s[k] = med3(dprev, nv, dcur);  // old s[k] was last read by dcur
+    dprev = dcur;
+  }
+}
+
+// Rows [i0, i1) of one curve (sample i at xq[i*stride]) by the group of G
+// lanes this lane (position gp) belongs to; every lane of the group runs the
+// same rows, and lane G/2 - 1 writes the outputs.
+template <int WL, int G>
+__device__ __forceinline__ void medfilt_group_run(const float *__restrict__ xq,
+                                                  float *__restrict__ yq, size_t stride, int n,
+                                                  int i0, int i1, int r, int mode, int gp) {
+  constexpr int T = G * WL, RM = T / 2 - 1;
+  float s[WL];
+#pragma unroll
+  for (int k = 0; k < WL; ++k) {
+    const int gk = gp * WL + k;
+    s[k] = (gk < RM - r) ? -INFINITY : ((gk > RM + r) ? INFINITY : 0.0f);
+  }
+  auto ld = [&](int i) -> float { return (i >= 0 && i < n) ? xq[(size_t)i * stride] : 0.0f; };
+  for (int j = -r; j <= r; j += kUnroll) {
+    float v[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) v[u] = (j + u <= r) ? ld(i0 + j + u) : 0.0f;
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u)
+      if (j + u <= r) slide_group<WL, G>(s, 0.0f, v[u], gp);
+  }
+  const bool writer = gp == G / 2 - 1;
+  float cin[kUnroll], cout_[kUnroll], cc[kUnroll];
+#pragma unroll
+  for (int u = 0; u < kUnroll; ++u) {
+    cc[u] = ld(i0 + u);
+    cin[u] = ld(i0 + u + r + 1);
+    cout_[u] = ld(i0 + u - r);
+  }
+  for (int i = i0; i < i1; i += kUnroll) {
+    float nin[kUnroll], nout[kUnroll], nc[kUnroll];
+    const int in_ = i + kUnroll;
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const bool more = in_ < i1;
+      nc[u] = more ? ld(in_ + u) : 0.0f;
+      nin[u] = more ? ld(in_ + u + r + 1) : 0.0f;
+      nout[u] = more ? ld(in_ + u - r) : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      if (i + u < i1) {
+        if (writer) {
+          const float med = s[WL - 1];
+          yq[(size_t)(i + u) * stride] = mode ? cc[u] / med : cc[u] - med;
+        }
+        slide_group<WL, G>(s, cout_[u], cin[u], gp);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      cc[u] = nc[u];
+      cin[u] = nin[u];
+      cout_[u] = nout[u];
+    }
+  }
+}
+
+// dense image: group g = segment*cols + curve, lanes g*G .. g*G + G-1
+template <int WL, int G>
+__global__ __launch_bounds__(256) void medfilt_group_kernel(
+    const float *__restrict__ x, float *__restrict__ y, int n, int nq, int r, int mode,
+    int rows_out, int cols_out, int seg, int nseg) {
+  const long lane = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long g = lane / G;
+  const int gp = (int)(lane % G);
+  if (g >= (long)nseg * cols_out) return;  // whole groups (blockDim is a multiple of G)
+  const int q = (int)(g % cols_out);
+  const int sidx = (int)(g / cols_out);
+  const int i0 = sidx * seg;
+  const int i1 = min(i0 + seg, rows_out);
+  if (i0 >= i1) return;
+  medfilt_group_run<WL, G>(x + q, y + q, (size_t)nq, n, i0, i1, r, mode, gp);
+}
+
+// ragged batch: group g filters segment g of the segment list
+template <int WL, int G>
+__global__ __launch_bounds__(256) void medfilt_ragged_group_kernel(
+    const float *__restrict__ x, float *__restrict__ y, const int64_t *__restrict__ off,
+    const int32_t *__restrict__ len, const int32_t *__restrict__ seg_curve,
+    const int32_t *__restrict__ seg_start, const int32_t *__restrict__ nseg_dev, int r, int mode,
+    int seg) {
+  const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = lane / G, gp = lane % G;
+  if (g >= *nseg_dev) return;
+  const int c = seg_curve[g];
+  const int n = len[c];
+  const int i0 = seg_start[g];
+  const int i1 = min(i0 + seg, n);
+  if (i0 >= i1) return;
+  medfilt_group_run<WL, G>(x + off[c], y + off[c], 1, n, i0, i1, r, mode, gp);
+}
+
 // Order-preserving map float -> uint32 (total order, -0 < +0).
 __device__ __forceinline__ uint32_t fkey(float v) {
   uint32_t b = __float_as_uint(v);
@@ -150,8 +285,9 @@ __device__ __forceinline__ float fkey_inv(uint32_t k) {
   return __uint_as_float(b);
 }
 
-// Wide windows (r > 64): exact (r+1)-th smallest by bisection over the 32-bit
-// key space, 32 counting passes over the window per output.
+// Windows beyond the lane groups (r > 1031): exact (r+1)-th smallest by
+// bisection over the 32-bit key space, 32 counting passes over the window per
+// output.
 __global__ __launch_bounds__(256) void medfilt_wide_kernel(
     const float *__restrict__ x, float *__restrict__ y, int n, int nq, int r, int mode,
     int rows_out, int cols_out) {
@@ -288,6 +424,17 @@ static void launch_win(const float *in, float *out, int n, int nq, int r, int mo
                      out, n, nq, r, mode, rows_out, cols_out, seg, nseg);
 }
 
+// Windows wider than the one-lane buckets (r > 64): groups of G lanes of
+// kGroupWL slots each, the smallest G in {2, 4, 8, 16} with r <= G*kGroupWL/2 - 1;
+// wider windows take the LDS bisection.
+constexpr int kGroupWL = 129;
+constexpr int kGroupMaxR = 16 * kGroupWL / 2 - 1;
+static int group_lanes(int r) {
+  int G = 2;
+  while (G < 16 && r > G * kGroupWL / 2 - 1) G <<= 1;
+  return G;
+}
+
 // Register-window buckets: the smallest RMAX >= r is used.
 static const int kBuckets[] = {4, 8, 16, 24, 32, 48, 64};
 
@@ -302,6 +449,33 @@ int medfilt_launch(const float *in, float *out, uint32_t n, uint32_t nq, uint32_
   GTRAP_REQUIRE(in != out, "gtrap_medfilt: in-place filtering is not supported");
   if (rows_out == 0 || cols_out == 0) return GTRAP_OK;
   const int w = 2 * (int)r + 1;
+  if (r > 64 && r <= (uint32_t)kGroupMaxR) {
+    const int G = group_lanes((int)r);
+    int seg = (int)seg_len;
+    if (seg <= 0) {
+      // 131072 lanes (eight waves per CU, the two per SIMD that 193 VGPRs
+      // allow: one full round), at least 4 windows per segment
+      const long target = 131072 / G;
+      long per = (long)cols_out >= target ? 1 : (target + cols_out - 1) / cols_out;
+      seg = (int)(((long)rows_out + per - 1) / per);
+      if (seg < 4 * w) seg = 4 * w;
+    }
+    if (seg > (int)rows_out) seg = (int)rows_out;
+    const int nseg = ((int)rows_out + seg - 1) / seg;
+    const long lanes = (long)nseg * cols_out * G;
+    const dim3 grid((unsigned)((lanes + 255) / 256));
+#define GTRAP_MF_GROUP(GG)                                                                      \
+  hipLaunchKernelGGL((medfilt_group_kernel<kGroupWL, GG>), grid, dim3(256), 0, st, in, out,      \
+                     (int)n, (int)nq, (int)r, mode, (int)rows_out, (int)cols_out, seg, nseg)
+    switch (G) {
+      case 2: GTRAP_MF_GROUP(2); break;
+      case 4: GTRAP_MF_GROUP(4); break;
+      case 8: GTRAP_MF_GROUP(8); break;
+      default: GTRAP_MF_GROUP(16); break;
+    }
+#undef GTRAP_MF_GROUP
+    return check_launch("medfilt_group_kernel");
+  }
   if (r > 64 && r <= (uint32_t)kWideMaxR && cols_out <= 65535) {
     const size_t shm = (size_t)(kWideTile + 2 * r) * sizeof(uint32_t);
     dim3 grid((rows_out + kWideTile - 1) / kWideTile, cols_out);
@@ -370,6 +544,21 @@ int medfilt_ragged_launch(const float *x, float *y, const int64_t *off, const in
   GTRAP_REQUIRE(r <= (uint32_t)kWideMaxR, "gtrap_medfilt_ragged: r must be <= %d", kWideMaxR);
   GTRAP_REQUIRE(seg >= 1 && max_segs >= 0, "gtrap_medfilt_ragged: bad segment shape");
   if (max_segs == 0) return GTRAP_OK;
+  if (r > 64 && r <= (uint32_t)kGroupMaxR) {
+    const int G = group_lanes((int)r);
+    const dim3 grid((unsigned)(((long)max_segs * G + 255) / 256));
+#define GTRAP_MF_RGROUP(GG)                                                                      \
+  hipLaunchKernelGGL((medfilt_ragged_group_kernel<kGroupWL, GG>), grid, dim3(256), 0, st, x, y,   \
+                     off, len, sc, ss, nseg, (int)r, mode, seg)
+    switch (G) {
+      case 2: GTRAP_MF_RGROUP(2); break;
+      case 4: GTRAP_MF_RGROUP(4); break;
+      case 8: GTRAP_MF_RGROUP(8); break;
+      default: GTRAP_MF_RGROUP(16); break;
+    }
+#undef GTRAP_MF_RGROUP
+    return check_launch("medfilt_ragged_group_kernel");
+  }
   if (r > 64) {
     const size_t shm = (size_t)(kWideTile + 2 * r) * sizeof(uint32_t);
     hipLaunchKernelGGL(medfilt_ragged_wide_kernel, dim3((unsigned)max_segs), dim3(kWideTile), shm,

@@ -46,11 +46,13 @@ def test_parity_shapes_and_edges(gpu, shape, r):
 
 
 @pytest.mark.parametrize("seg_len", [1, 7, 100, 5000])
-def test_segmentation_invariance(gpu, seg_len):
+@pytest.mark.parametrize("r", [16, 100])
+def test_segmentation_invariance(gpu, seg_len, r):
+    """One-lane windows (r = 16) and windows shared by a lane group (r = 100)."""
     rng = np.random.default_rng(3)
     img = (1.0 + 1e-3 * rng.standard_normal((777, 9))).astype(np.float32)
-    got = _run(img, 16, "sub", seg_len=seg_len)
-    want = om.detrend_scipy(img, 16, "sub")
+    got = _run(img, r, "sub", seg_len=seg_len)
+    want = om.detrend_scipy(img, r, "sub")
     np.testing.assert_array_equal(_bits(got), _bits(want))
 
 
@@ -128,11 +130,13 @@ def test_detrend_image_full_size_subset(gpu):
         np.testing.assert_array_equal(_bits(got), _bits(want), err_msg=f"r={r}")
 
 
-@pytest.mark.parametrize("r", [65, 100, 128, 512])
+@pytest.mark.parametrize("r", [65, 100, 128, 129, 200, 512, 700, 1031, 1032, 1500])
 def test_wide_windows_of_the_reference_drivers(gpu, r):
     """The r of kepler/gtls_mockkepler.py:229 (128), kepler/gtls_pickkepler.py:197
-    (100) and works/sp/gtls_tesslc.py:129 (512), on LDS tiles: bit-exact
-    against scipy.signal.medfilt, sub and div, with outliers and edges."""
+    (100) and works/sp/gtls_tesslc.py:129 (512), and the edges of every lane
+    group size (2, 4, 8, 16 lanes up to r = 128, 257, 515, 1031; the LDS
+    bisection above): bit-exact against scipy.signal.medfilt, sub and div, with
+    outliers and edges."""
     import torch
     from gtrap_amd import gfilter
     rng = np.random.default_rng(r)

@@ -55,7 +55,7 @@ def test_ragged_detrend_bit_exact(gpu):
     from gtrap_amd import _native
     rng = np.random.default_rng(5)
     lens = [7, 64, 129, 1000, 5000, 20011]
-    for r in (7, 24, 64, 100, 128, 512):
+    for r in (7, 24, 64, 100, 128, 512, 700, 1500):
         xs = [(1 + 1e-3 * rng.standard_normal(n)).astype(np.float32) for n in lens]
         off = np.concatenate([[0], np.cumsum(lens[:-1])]).astype(np.int64)
         flat = np.concatenate(xs)
