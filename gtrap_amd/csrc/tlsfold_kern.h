@@ -571,11 +571,18 @@ __device__ void tf_stage(TfLds &L, const TfRec *rec, int nv, double P, int lo,
     for (int v = 0; v < 5; ++v) L.scan[wv][v] = inc[v];
   }
   __syncthreads();
+  // the totals of the waves before this one, added in wave order; every
+  // wave's total is read at once (a loop to wv would chain up to 35 LDS round
+  // trips), the select keeps the sum's order and bits
   double run[5];
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
+    double tot[kTfThreads / kWave];
+#pragma unroll
+    for (int w = 0; w < kTfThreads / kWave; ++w) tot[w] = L.scan[w][v];
     double base = 0.0;
-    for (int w = 0; w < wv; ++w) base += L.scan[w][v];
+#pragma unroll
+    for (int w = 0; w < kTfThreads / kWave; ++w) base = w < wv ? base + tot[w] : base;
     run[v] = base + inc[v] - acc[v];  // exclusive prefix of this thread
   }
 #pragma unroll
@@ -958,7 +965,8 @@ __global__ __launch_bounds__(kTfThreads) void tlsfold_kernel(TfArgs A) {
       if (lane == kWave - 1) s_wsum[wv] = inc;
       __syncthreads();
       int run = inc - sum;
-      for (int w = 0; w < wv; ++w) run += s_wsum[w];
+#pragma unroll
+      for (int w = 0; w < kTfThreads / kWave; ++w) run += w < wv ? s_wsum[w] : 0;
       for (int k = a0; k < b0; ++k) { const int v = lcur[k]; lcur[k] = run; run += v; }
       if (tid == 0) lcur[nb] = nv;
       __syncthreads();
