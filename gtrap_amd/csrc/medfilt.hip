@@ -492,9 +492,13 @@ int medfilt_launch(const float *in, float *out, uint32_t n, uint32_t nq, uint32_
   }
   int seg = (int)seg_len;
   if (seg <= 0) {
-    // Enough lanes for 256 CUs x 4 SIMDs x 4 waves, but at least 4 windows per
-    // segment so the initial fill is amortised.
-    const long target = 262144;
+    // One round of waves (256 CUs x 4 SIMDs x the waves per SIMD that the
+    // register window allows: 4 up to RMAX = 32, 2 above), at least 4 windows
+    // per segment so the initial fill is amortised.
+    int rmax_ = 64;
+    for (int b : kBuckets)
+      if ((int)r <= b) { rmax_ = b; break; }
+    const long target = rmax_ >= 48 ? 131072 : 262144;
     long per = (long)cols_out >= target ? 1 : (target + cols_out - 1) / cols_out;
     seg = (int)(((long)rows_out + per - 1) / per);
     if (seg < 4 * w) seg = 4 * w;
