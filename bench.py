@@ -130,7 +130,8 @@ def bench_gfilter(steps: int, warmup: int):
                                  n=n, nq=nq, radii=list(radii)),
         roofline=dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
                       frac=ach / HBM_PEAK_GBS, traffic=None,
-                      kernel=f"medfilt_win_kernel r={dom}",
+                      kernel=(f"medfilt_block_kernel r={dom}" if dom in (24, 64)
+                              else f"medfilt_win_kernel r={dom}"),
                       kernel_ms={str(r): kms[r] for r in radii}),
         wide_windows=wide,
         cpu_baseline=dict(value=cpu_rate, unit="light curves/s", cores=1, kind="port",

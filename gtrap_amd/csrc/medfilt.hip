@@ -17,12 +17,18 @@
 // contiguous bytes per row (coalesced), unlike the reference's (1,1000)
 // blocks that read one curve per block with stride nq.
 //
+// The batch workload's radii r = 24 and r = 64 run a blocked form instead:
+// the window rows shared by B consecutive outputs are ordered once by a
+// sorting network and each output's median is selected from them and its
+// B-1 own rows (fewer VALU ops per output than sliding the whole window).
+//
 // Windows wider than 129 slots (r > 64, the reference drivers' r = 100, 128
 // and 512) keep one sorted window across a group of 2-16 lanes (129 slots
 // each, DPP moves between neighbours); windows beyond 16 lanes (r > 1031) use
 // an exact bisection over the float key space on LDS tiles.
 #include "common.h"
 #include <math.h>
+#include <utility>
 
 namespace gtrap {
 
@@ -143,6 +149,200 @@ __global__ __launch_bounds__(256) void medfilt_ragged_kernel(
   const int i1 = min(i0 + seg, n);
   if (i0 >= i1) return;
   medfilt_run<RMAX>(x + off[c], y + off[c], 1, n, i0, i1, r, mode);
+}
+
+// ---- blocked windows (r = 24 and r = 64): B outputs per sorted core ----
+//
+// The windows (W = 2r + 1 rows) of the B consecutive outputs i .. i+B-1 share
+// M = W - (B-1) rows, i+B-1-r .. i+r, the core C.  Output j adds B-1 extra
+// rows, E_j = rows i+j-r .. i+B-2-r and i+r+1 .. i+r+j.  With C and E_j each
+// sorted, the median (the (r+1)-th smallest of C u E_j) is
+//   min over a = 0..B-1 of max(E_j[a-1], C[r-a])     (E_j[-1] = -inf):
+// every term is at least the (r+1)-th smallest (a elements of E_j and r+1-a
+// of C lie at or below it) and the term with a = the number of E_j elements
+// among the r+1 smallest equals it.  Only C[r-B+1 .. r] enter, so per block
+// the core is ordered by a sorting network pruned to those ranks, E_0 by a
+// full one, and E_{j+1} follows from E_j by the one-lane slide (delete row
+// i+j-r, insert row i+r+1+j).  Per output that is 2(B-1) min/max for the
+// selection, 3(B-1) for the slide and the networks' share: at r = 64, B = 24
+// (1030 + 127 compare-exchanges per block) about 220 VALU ops against the
+// slide's 3 W = 387 over the whole window; at r = 24, B = 16 (232 + 59)
+// about 115 against 147.
+
+// Compare-exchange list of Batcher's odd-even merge sort over the next power
+// of two >= m, less the exchanges that are no-ops or dead: padding slots
+// >= m hold +inf, which every exchange (a, b), a < b, leaves in place, so
+// exchanges with b >= m are dropped; exchanges whose outputs reach none of
+// the ranks lo..hi (backwards from the end) are dropped too.
+struct MfNet {
+  int n;
+  unsigned char a[1600], b[1600];
+};
+constexpr MfNet mf_make_net(int m, int lo, int hi) {
+  MfNet all{0, {}, {}};
+  int np2 = 1;
+  while (np2 < m) np2 <<= 1;
+  for (int p = 1; p < np2; p <<= 1)
+    for (int k = p; k >= 1; k >>= 1)
+      for (int j = k % p; j + k < np2; j += 2 * k)
+        for (int i = 0; i < k && i < np2 - j - k; ++i)
+          if ((i + j) / (2 * p) == (i + j + k) / (2 * p) && i + j + k < m) {
+            all.a[all.n] = (unsigned char)(i + j);
+            all.b[all.n] = (unsigned char)(i + j + k);
+            ++all.n;
+          }
+  bool live[256] = {};
+  for (int k = lo; k <= hi; ++k) live[k] = true;
+  bool keep[1600] = {};
+  int nk = 0;
+  for (int c = all.n - 1; c >= 0; --c)
+    if (live[all.a[c]] || live[all.b[c]]) {
+      keep[c] = true;
+      live[all.a[c]] = live[all.b[c]] = true;
+      ++nk;
+    }
+  MfNet net{0, {}, {}};
+  for (int c = 0; c < all.n; ++c)
+    if (keep[c]) {
+      net.a[net.n] = all.a[c];
+      net.b[net.n] = all.b[c];
+      ++net.n;
+    }
+  return net;
+}
+template <int M, int LO, int HI>
+struct MfNetT {
+  static constexpr MfNet net = mf_make_net(M, LO, HI);
+  static_assert(net.n > 0 && net.n <= 1600, "sorting network table");
+};
+
+// min and max as v_med3_f32 against -inf/+inf: the same values as fminf/fmaxf
+// for non-NaN operands, without the canonicalising v_max_f32 that IEEE-mode
+// fminf/fmaxf put in front of operands read from memory.  The infinities come
+// from scalar moves the compiler cannot see through (it would fold med3
+// against a constant infinity back into fminf/fmaxf).
+struct MfInf {
+  float lo, hi;
+};
+__device__ __forceinline__ MfInf mf_inf() {
+  MfInf f;
+  asm("s_mov_b32 %0, 0xff800000" : "=s"(f.lo));
+  asm("s_mov_b32 %0, 0x7f800000" : "=s"(f.hi));
+  return f;
+}
+__device__ __forceinline__ float mf_min(float x, float y, MfInf f) { return med3(x, y, f.lo); }
+__device__ __forceinline__ float mf_max(float x, float y, MfInf f) { return med3(x, y, f.hi); }
+
+template <int A, int B, int N>
+__device__ __forceinline__ void mf_ce(float (&v)[N], MfInf f) {
+  const float x = v[A], y = v[B];
+  v[A] = mf_min(x, y, f);
+  v[B] = mf_max(x, y, f);
+}
+template <class T, int N, size_t... C>
+__device__ __forceinline__ void mf_apply(float (&v)[N], MfInf f, std::index_sequence<C...>) {
+  (mf_ce<T::net.a[C], T::net.b[C]>(v, f), ...);
+}
+
+// A float 0.0 in device memory: rows outside the curve read it (the zero
+// padding) through the same load as every other row, so edge blocks issue
+// their loads together instead of branching around them.
+__device__ float mf_zero_row = 0.0f;
+
+// Rows [i0, i1) of one curve (sample i at xq[i*stride]), r = RM (no
+// sentinels).  Blocks whose rows i-RM .. i+RM+B all lie inside the curve read
+// through running pointers (one 64-bit add per row); blocks at the curve's
+// ends read row by row through a bounds check that points outside rows at
+// mf_zero_row.
+template <int RM, int B>
+__device__ __forceinline__ void medfilt_block_run(const float *__restrict__ xq,
+                                                  float *__restrict__ yq, size_t stride, int n,
+                                                  int i0, int i1, int mode) {
+  constexpr int W = 2 * RM + 1, M = W - (B - 1), NE = B - 1;
+  static_assert(B % kUnroll == 0, "whole load groups per block");
+  using NetC = MfNetT<M, RM - NE, RM>;
+  using NetE = MfNetT<NE, 0, NE - 1>;
+  const float *zero = &mf_zero_row;
+  const MfInf inf = mf_inf();
+  // row k's address, or the zero row outside the curve
+  auto at = [&](int k) -> const float * {
+    return (unsigned)k < (unsigned)n ? xq + (size_t)(unsigned)k * stride : zero;
+  };
+  for (int i = i0; i < i1; i += B) {
+    const bool inner = i - RM >= 0 && i + RM + B < n;  // wave-uniform for a dense image
+    // rows i+jg .. +kUnroll-1 (outputs), RM before them (deleted from E) and
+    // RM+1 after them (inserted)
+    auto load_group = [&](int jg, float (&xc)[kUnroll], float (&xo)[kUnroll],
+                          float (&xn)[kUnroll]) {
+      if (inner) {
+        const float *pc = xq + (size_t)(i + jg) * stride, *po = pc - (size_t)RM * stride,
+                    *pn = pc + (size_t)(RM + 1) * stride;
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u, pc += stride, po += stride, pn += stride) {
+          xc[u] = *pc;
+          xo[u] = *po;
+          xn[u] = *pn;
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+          const int j = i + jg + u;
+          xc[u] = *at(j);
+          xo[u] = *at(j - RM);
+          xn[u] = *at(j + RM + 1);
+        }
+      }
+    };
+    // core slot s holds row i+B-1-RM+s; E_0 rows i-RM .. i+B-2-RM
+    float c[M], e[NE];
+    if (inner) {
+      const float *p = xq + (size_t)(i - RM) * stride;
+#pragma unroll
+      for (int t = 0; t < NE; ++t, p += stride) e[t] = *p;
+#pragma unroll
+      for (int s = 0; s < M; ++s, p += stride) c[s] = *p;
+    } else {
+#pragma unroll
+      for (int t = 0; t < NE; ++t) e[t] = *at(i - RM + t);
+#pragma unroll
+      for (int s = 0; s < M; ++s) c[s] = *at(i + B - 1 - RM + s);
+    }
+    mf_apply<NetC>(c, inf, std::make_index_sequence<NetC::net.n>{});
+    float cm[B];  // C[RM-B+1 .. RM]
+#pragma unroll
+    for (int t = 0; t < B; ++t) cm[t] = c[RM - NE + t];
+    mf_apply<NetE>(e, inf, std::make_index_sequence<NetE::net.n>{});
+    // outputs in groups of kUnroll, the group's rows loaded before its first use
+    float *py = yq + (size_t)i * stride;
+    for (int jg = 0; jg < B; jg += kUnroll) {
+      float xc[kUnroll], xo[kUnroll], xn[kUnroll];
+      load_group(jg, xc, xo, xn);
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u, py += stride) {
+        float med = cm[NE];
+#pragma unroll
+        for (int k = 1; k <= NE; ++k) med = mf_min(med, mf_max(e[k - 1], cm[NE - k], inf), inf);
+        if (i + jg + u < i1) *py = mode ? xc[u] / med : xc[u] - med;
+        slide_sorted<NE>(e, xo[u], xn[u]);  // E_{j+1} (unused after the block's last output)
+      }
+    }
+  }
+}
+
+// dense image, r = RM: lane g = segment*cols + curve filters rows
+// [segment*seg, +seg) (seg a multiple of B)
+template <int RM, int B>
+__global__ __launch_bounds__(256) void medfilt_block_kernel(
+    const float *__restrict__ x, float *__restrict__ y, int n, int nq, int mode, int rows_out,
+    int cols_out, int seg, int nseg) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (long)nseg * cols_out) return;
+  const int q = (int)(g % cols_out);
+  const int sidx = (int)(g / cols_out);
+  const int i0 = sidx * seg;
+  const int i1 = min(i0 + seg, rows_out);
+  if (i0 >= i1) return;
+  medfilt_block_run<RM, B>(x + q, y + q, (size_t)nq, n, i0, i1, mode);
 }
 
 // ---- wide windows in registers: one sorted window over a group of G lanes ----
@@ -435,7 +635,12 @@ static int group_lanes(int r) {
   return G;
 }
 
-// Register-window buckets: the smallest RMAX >= r is used.
+// Register-window buckets: the smallest RMAX >= r is used.  r = 24 and r = 64
+// themselves (the detrend radii of the batch workload) run the blocked
+// selection with kBlockLanes lanes: 1.5 rounds of waves at r = 64 (156 VGPRs,
+// three waves per SIMD), about one at r = 24 (96 VGPRs, five); measured
+// within 3% of the best of 196608 .. 524288 lanes for both.
+constexpr long kBlockLanes = 393216;
 static const int kBuckets[] = {4, 8, 16, 24, 32, 48, 64};
 
 int medfilt_launch(const float *in, float *out, uint32_t n, uint32_t nq, uint32_t r, int mode,
@@ -489,6 +694,28 @@ int medfilt_launch(const float *in, float *out, uint32_t n, uint32_t nq, uint32_
                        st, in, out, (int)n, (int)nq, (int)r, mode, (int)rows_out,
                        (int)cols_out);
     return check_launch("medfilt_wide_kernel");
+  }
+  if ((int)r == 64 || (int)r == 24) {
+    // blocked selection (B = 24 outputs per core at r = 64, 16 at r = 24):
+    // kBlockLanes lanes, segments a whole number of blocks
+    const int B = (int)r == 64 ? 24 : 16;
+    int seg = (int)seg_len;
+    if (seg <= 0) {
+      const long per = (long)cols_out >= kBlockLanes ? 1 : (kBlockLanes + cols_out - 1) / cols_out;
+      seg = (int)(((long)rows_out + per - 1) / per);
+      seg = (seg + B - 1) / B * B;
+    }
+    if (seg > (int)rows_out) seg = (int)rows_out;
+    const int nseg = ((int)rows_out + seg - 1) / seg;
+    const long lanes = (long)nseg * cols_out;
+    const dim3 grid((unsigned)((lanes + 255) / 256));
+    if ((int)r == 64)
+      hipLaunchKernelGGL((medfilt_block_kernel<64, 24>), grid, dim3(256), 0, st, in, out, (int)n,
+                         (int)nq, mode, (int)rows_out, (int)cols_out, seg, nseg);
+    else
+      hipLaunchKernelGGL((medfilt_block_kernel<24, 16>), grid, dim3(256), 0, st, in, out, (int)n,
+                         (int)nq, mode, (int)rows_out, (int)cols_out, seg, nseg);
+    return check_launch("medfilt_block_kernel");
   }
   int seg = (int)seg_len;
   if (seg <= 0) {

@@ -35,7 +35,7 @@ def test_parity_random(gpu, r, mode):
 
 
 @pytest.mark.parametrize("shape", [(513, 1), (64, 3), (5, 2), (1, 1), (300, 130)])
-@pytest.mark.parametrize("r", [3, 24, 70])
+@pytest.mark.parametrize("r", [3, 24, 64, 70])
 def test_parity_shapes_and_edges(gpu, shape, r):
     rng = np.random.default_rng(sum(shape) + r)
     img = rng.integers(-4, 5, size=shape).astype(np.float32)  # heavy ties
@@ -46,9 +46,11 @@ def test_parity_shapes_and_edges(gpu, shape, r):
 
 
 @pytest.mark.parametrize("seg_len", [1, 7, 100, 5000])
-@pytest.mark.parametrize("r", [16, 100])
+@pytest.mark.parametrize("r", [16, 64, 100])
 def test_segmentation_invariance(gpu, seg_len, r):
-    """One-lane windows (r = 16) and windows shared by a lane group (r = 100)."""
+    """One-lane windows (r = 16), the blocked selection (r = 64: segments that
+    are and are not whole 32-row blocks, blocks at the curve's ends) and
+    windows shared by a lane group (r = 100)."""
     rng = np.random.default_rng(3)
     img = (1.0 + 1e-3 * rng.standard_normal((777, 9))).astype(np.float32)
     got = _run(img, r, "sub", seg_len=seg_len)
@@ -56,11 +58,12 @@ def test_segmentation_invariance(gpu, seg_len, r):
     np.testing.assert_array_equal(_bits(got), _bits(want))
 
 
-def test_partial_coverage_leaves_rest_untouched(gpu):
+@pytest.mark.parametrize("r", [10, 64])
+def test_partial_coverage_leaves_rest_untouched(gpu, r):
     rng = np.random.default_rng(4)
     img = (1.0 + 1e-3 * rng.standard_normal((1000, 8))).astype(np.float32)
-    got = _run(img, 10, "sub", rows_out=900, cols_out=6, fill=-7.0)
-    want = om.detrend_scipy(img, 10, "sub")
+    got = _run(img, r, "sub", rows_out=900, cols_out=6, fill=-7.0)
+    want = om.detrend_scipy(img, r, "sub")
     np.testing.assert_array_equal(_bits(got[:900, :6]), _bits(want[:900, :6]))
     assert np.all(got[900:] == -7.0) and np.all(got[:, 6:] == -7.0)
 
