@@ -43,6 +43,7 @@ SIGNATURES: dict[str, tuple] = {
     "gtrap_gtls": (_int, [_vp] * 7 + [_i32] * 5 + [_f32] * 5 + [_int, _vp]),
     "gtrap_tlsgrid": (_int, [_vp, _vp, _i32, _vp, _i32, _vp, _i32, _vp, _i32, _int, _f64, _i32,
                              _vp, _vp, _vp, _vp, _vp]),
+    "gtrap_tlsfit_best": (_int, [_vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp]),
     "gtrap_tlsfold_ws_bytes": (_i64, [_i32, _i32]),
     "gtrap_tlsfold": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _i32,
                              _i64, _vp, _i32, _i32, _f64, _i32, _vp, _i64, _i32, _vp]),

@@ -142,7 +142,64 @@ __global__ __launch_bounds__(kTgThreads) void tlsgrid_kernel(
   }
 }
 
+// tlsfit's choice over the grid (tls.py:100-110): sn = -H / sqrt(res/(n-3))
+// per trial in float64 as numpy evaluates it (n - 3 in int32, then the true
+// division), and numpy's nanargmax: NaN counts as -inf, the first maximum
+// wins, and a grid of NaN only has no choice (index -1).  One workgroup;
+// each thread keeps the first maximum of its trials (in index order), then
+// the threads' maxima meet by value, ties to the lower index.
+constexpr int kTbThreads = 1024;
+__global__ __launch_bounds__(kTbThreads) void tlsfit_best_kernel(
+    const double *__restrict__ H, const double *__restrict__ res, const int32_t *__restrict__ nn,
+    const double *__restrict__ xs, int32_t ntr, double *__restrict__ sn_out,
+    double *__restrict__ best) {
+  __shared__ double s_v[kTbThreads / kWave];
+  __shared__ int s_i[kTbThreads / kWave];
+  __shared__ int s_any[kTbThreads / kWave];
+  double bv = -INFINITY;
+  int bi = -1, any = 0;
+  for (int i = threadIdx.x; i < ntr; i += kTbThreads) {
+    const double s = -H[i] / sqrt(res[i] / (double)(nn[i] - 3));
+    if (sn_out) sn_out[i] = s;
+    any |= isnan(s) ? 0 : 1;
+    const double v = isnan(s) ? -INFINITY : s;
+    if (bi < 0 || v > bv) { bv = v; bi = i; }
+  }
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    const double ov = __shfl_xor(bv, o);
+    const int oi = __shfl_xor(bi, o);
+    any |= __shfl_xor(any, o);
+    if (oi >= 0 && (bi < 0 || ov > bv || (ov == bv && oi < bi))) { bv = ov; bi = oi; }
+  }
+  const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+  if (lane == 0) { s_v[wv] = bv; s_i[wv] = bi; s_any[wv] = any; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kTbThreads / kWave; ++w) {
+      const double ov = s_v[w];
+      const int oi = s_i[w];
+      any |= s_any[w];
+      if (oi >= 0 && (bi < 0 || ov > bv || (ov == bv && oi < bi))) { bv = ov; bi = oi; }
+    }
+    if (!any) bi = -1;
+    best[0] = (double)bi;
+    best[1] = bi >= 0 ? -H[bi] / sqrt(res[bi] / (double)(nn[bi] - 3)) : NAN;
+    best[2] = bi >= 0 ? H[bi] : NAN;
+    best[3] = bi >= 0 && xs ? xs[bi] : 0.0;
+  }
+}
+
 }  // namespace gtrap
+
+extern "C" int gtrap_tlsfit_best(const double *H, const double *res, const int32_t *nout,
+                                 const double *xsum, int32_t ntr, double *sn, double *best,
+                                 void *stream) {
+  GTRAP_REQUIRE(H && res && nout && best, "gtrap_tlsfit_best: null pointer");
+  GTRAP_REQUIRE(ntr >= 0, "gtrap_tlsfit_best: negative size");
+  hipLaunchKernelGGL(gtrap::tlsfit_best_kernel, dim3(1), dim3(gtrap::kTbThreads), 0,
+                     gtrap::as_stream(stream), H, res, nout, xsum, ntr, sn, best);
+  return gtrap::check_launch("tlsfit_best_kernel");
+}
 
 extern "C" int gtrap_tlsgrid(const double *t, const double *x, int32_t n, const double *Wl,
                              int32_t nW, const double *Ll, int32_t nL, const double *tl,

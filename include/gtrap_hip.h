@@ -294,6 +294,18 @@ int gtrap_tlsgrid(const double *t, const double *x, int32_t n, const double *Wl,
                   double sigma, int32_t sorted, double *H, double *res, int32_t *nout,
                   double *xsum, void *stream);
 
+/*
+ * tlsfit's choice over a gtrap_tlsgrid grid (replaces gtrap/tls.py:100-110:
+ * sn = -H/sqrt(res/(n-3)), then np.nanargmax): sn per trial in float64 as
+ * numpy evaluates it (written to sn when it is not NULL), and best[0..3] =
+ * {index of the first maximum with NaN counted as -inf, its sn, its H, its
+ * xsum (0 when xsum is NULL)}; best[0] = -1 when every sn is NaN (numpy
+ * raises there).  All pointers are device memory of ntr trials, best of 4
+ * doubles; one workgroup, asynchronous on stream.
+ */
+int gtrap_tlsfit_best(const double *H, const double *res, const int32_t *nout,
+                      const double *xsum, int32_t ntr, double *sn, double *best, void *stream);
+
 /* ---- periodic trapezoid least squares (tls_fold) --------------------------- */
 #define GTRAP_TLS_SNR   0  /* tls.py's S/N = -H / sqrt(res/(n-3)) (tls.py:100) */
 #define GTRAP_TLS_DCHI2 1  /* least-squares detection statistic, see gtrap_tlsfold */

@@ -71,3 +71,97 @@ def test_window_search_and_unsorted_time(gpu):
                 assert a[2][w, l, k] == no
                 np.testing.assert_allclose([a[0][w, l, k], a[1][w, l, k], a[3][w, l, k]],
                                            [Ho, ro, xo], rtol=1e-9, atol=1e-14)
+
+
+def _best(H, res, n, xs=None, with_sn=True):
+    import torch
+    from gtrap_amd import _native
+    d = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a, dt)).cuda()
+    Hd, rd, nd = d(H, np.float64), d(res, np.float64), d(n, np.int32)
+    xd = d(xs, np.float64) if xs is not None else None
+    sn = torch.full((len(H),), 7.0, dtype=torch.float64, device="cuda")
+    best = torch.zeros(4, dtype=torch.float64, device="cuda")
+    p = _native.ptr
+    _native.call("gtrap_tlsfit_best", p(Hd), p(rd), p(nd), p(xd), len(H),
+                 p(sn) if with_sn else None, p(best), _native.stream_handle())
+    torch.cuda.synchronize()
+    return sn.cpu().numpy(), best.cpu().numpy()
+
+
+def _numpy_choice(H, res, n):
+    with np.errstate(all="ignore"):
+        sn = -H / np.sqrt(res / (n.astype(np.int32) - 3))
+    try:
+        return sn, int(np.nanargmax(sn))
+    except ValueError:
+        return sn, -1
+
+
+@pytest.mark.parametrize("case", ["random", "special", "ties", "neginf", "allnan", "one"])
+def test_tlsfit_choice_matches_numpy(gpu, case):
+    """tls.py:100-110: the device S/N equals numpy's bit for bit (true
+    division and a correctly rounded sqrt), and the index is np.nanargmax's
+    (NaN as -inf, first maximum); NaN only gives -1 (numpy raises)."""
+    rng = np.random.default_rng(5)
+    m = 5000
+    H = rng.normal(0, 1e-3, m)
+    res = rng.uniform(1e-8, 1e-3, m)
+    n = rng.integers(0, 400, m).astype(np.int32)
+    if case == "special":
+        H[:50] = 0.0
+        res[40:90] = 0.0
+        res[90:120] = -1e-6
+        n[120:200] = 3
+        n[200:220] = 2
+        H[220:230] = np.inf
+    elif case == "ties":
+        H[:] = -1e-3
+        res[:] = 1e-4
+        n[:] = 100
+        H[[4000, 17, 2500]] = -2e-3  # the same S/N three times: the first index wins
+    elif case == "neginf":
+        H[:] = np.nan
+        H[[3000, 1234]] = 1.0
+        res[[3000, 1234]] = 0.0  # -1/0 = -inf: the maximum is -inf, at the first of them
+        n[[3000, 1234]] = 10
+        # every S/N is NaN or -inf: nanargmax counts NaN as -inf and returns the
+        # first index, trial 0 (a NaN trial), and so must the device
+    elif case == "allnan":
+        H[:] = np.nan
+    elif case == "one":
+        H, res, n = H[:1], res[:1], n[:1]
+    xs = rng.normal(0, 1, len(H))
+    sn, best = _best(H, res, n, xs)
+    want_sn, want_i = _numpy_choice(H, res, n)
+    nan = np.isnan(want_sn)
+    np.testing.assert_array_equal(np.isnan(sn), nan)
+    np.testing.assert_array_equal(sn[~nan].view(np.uint64), want_sn[~nan].view(np.uint64))
+    assert int(best[0]) == want_i, (case, best[0], want_i)
+    if want_i >= 0:
+        np.testing.assert_array_equal(best[2:], [H[want_i], xs[want_i]])  # NaN == NaN here
+    # without the S/N array and without xsum: same choice
+    _, b2 = _best(H, res, n, None, with_sn=False)
+    assert int(b2[0]) == want_i and (want_i < 0 or b2[3] == 0.0)
+
+
+def test_tlsfit_repeated_refits_reuse_buffers(gpu):
+    """Refits of one curve on changing grids (the reference's refit loop,
+    works/sp/gtls_tesslc.py:299-314) through the reused buffers: each equals
+    the choice numpy makes on that grid's device outputs."""
+    from gtrap_amd import tls
+    rng = np.random.default_rng(8)
+    t = np.arange(4000) * (2.0 / 1440.0)
+    x = 1.0 + 5e-4 * rng.standard_normal(4000) + tls.gen_trapzoid(t, -2e-3, 0.3, 0.05, 3.1)
+    x[rng.random(4000) < 0.01] = np.nan
+    for c, nt in ((3.1, 10), (1.7, 7), (4.2, 13), (3.1, 10)):
+        tl = np.linspace(c - 0.1, c + 0.1, nt)
+        Wl = np.linspace(0.297, 0.303, 10)
+        Ll = np.linspace(0.0, 0.15, 20)
+        Hd, rd, nd, _ = (a.cpu().numpy() for a in tls.grid_device(t, x, Wl, Ll, tl))
+        with np.errstate(all="ignore"):
+            sn = -Hd / np.sqrt(rd / (nd - 3))
+        w, l, k = np.unravel_index(np.nanargmax(sn), sn.shape)
+        xpre, H, W, L, T0, off = tls.tlsfit(t, x, Wl, Ll, tl)
+        assert (W, L, T0, H) == (Wl[w], Ll[l], tl[k], Hd[w, l, k])
+        with np.errstate(all="ignore"):
+            np.testing.assert_array_equal(xpre, tls.gen_trapzoid(t, H, W, L, T0) + off)
