@@ -295,9 +295,11 @@ __device__ __forceinline__ void gf_first4(const GfLds &L, int e, int nsc, int to
 // failing the test from b on are a prefix of b, b+1, ... (the test is
 // monotone in the row, K being non-decreasing), so four rows tested at once
 // advance b by their number of failures, and another round runs only when
-// all four fail.  No dependent LDS round trip per row (a row-by-row walk in
-// both directions took 96.2 ms at the Kepler setting, this 86.9; two rows at
-// once 86.5, six 91.9, eight 101.2).
+// all four fail.  The four reads are issued together, unconditionally (rows
+// clamped into the scoop, the bounds applied to the test): a row-by-row walk
+// in both directions took 96.2 ms at the Kepler setting, this 75.5 (two rows
+// at once 78.1, three 75.6, six 80.0; bounds-checked reads, which the
+// compiler turns into one branch and one LDS round trip per row, 86.9).
 constexpr int kGfProbe = 4;
 __device__ __forceinline__ int gf_up(const GfLds &L, int e, int nsc, float mdk, float kf,
                                      float thr, int b) {
@@ -306,8 +308,8 @@ __device__ __forceinline__ int gf_up(const GfLds &L, int e, int nsc, float mdk, 
 #pragma unroll
     for (int j = 0; j < kGfProbe; ++j) {
       const int r = b + j;  // rows at or past nsc pass
-      const bool fail = r < nsc && !(fmaf(mdk, kf, L.K[e + min(r, nsc - 1)]) > thr);
-      f += fail ? 1 : 0;
+      const float ct = fmaf(mdk, kf, L.K[e + min(r, nsc - 1)]);  // every read issued
+      f += (int)((r < nsc) & !(ct > thr));
     }
     b += f;
     if (f < kGfProbe) return b;
@@ -322,8 +324,8 @@ __device__ __forceinline__ int gf_down(const GfLds &L, int e, int nsc, float mdk
 #pragma unroll
     for (int j = 1; j <= kGfProbe; ++j) {
       const int r = b - j;
-      const bool pass = r >= 0 && fmaf(mdk, kf, L.K[e + max(r, 0)]) > thr;
-      p += pass ? 1 : 0;
+      const float ct = fmaf(mdk, kf, L.K[e + max(r, 0)]);
+      p += (int)((r >= 0) & (ct > thr));
     }
     b -= p;
     if (p < kGfProbe) return b;
