@@ -502,10 +502,13 @@ __device__ void tf_stage(TfLds &L, const TfRec *rec, int nv, double P, int lo,
                          int cnt, double r, double xref, int cells) {
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
   const int k0 = tid * kPer;
-  // image of extended element j = lo + k of record value u
+  // image of extended element j = lo + k of record value u: u - P, u or
+  // u + P, as one fma with the shift -1, 0 or +1 (rounded exactly as the sum;
+  // a phase of -0 would come out +0, which no comparison or sum tells apart);
+  // selecting between both sums cost 1.2% more on the serving mix
   auto image = [&](int k, double u) {
     const int j = lo + k;
-    return j < 0 ? u - P : (j >= nv ? u + P : u);
+    return fma((double)((j >= nv) - (j < 0)), P, u);
   };
   double img[kPer], xs[kPer], qs[kPer];
   double acc[5] = {0, 0, 0, 0, 0};
