@@ -250,42 +250,34 @@ struct GfLds {
   double *P;      // prefix sums [R+1][5]: x', x'^2, q, q^2, x'q over valid rows
 };
 
-// first row j in [0, nsc) of the scoop at union row e with
-// fl32(K - deltat*k) > thr (ge = 0) or >= thr (ge = 1); nsc if none
-__device__ __forceinline__ int gf_first(const GfLds &L, int e, int nsc, float mdk, float kf,
-                                        float thr, int ge) {
-  int lo = 0, n = nsc;
-  while (n > 0) {
-    const int h = n >> 1;
-    const float ct = fmaf(mdk, kf, L.K[e + lo + h]);
-    const bool pass = ge ? (ct >= thr) : (ct > thr);
-    if (!pass) { lo += h + 1; n -= h + 1; } else n = h;
-  }
-  return lo;
-}
-
-// Four searches of gf_first (strict >) in lockstep with a fixed depth: every
-// step tests row lo + step - 1 of each (a row at or past nsc counts as
-// passing), so the four independent LDS reads of a step are in flight
-// together.  top = the smallest power of two >= nsc.
-__device__ __forceinline__ void gf_first4(const GfLds &L, int e, int nsc, int top, float mdk,
-                                          float kf, const float *thr, int *out) {
-  int lo[4] = {0, 0, 0, 0};
-  for (int step = top; step > 0; step >>= 1) {
-    float c[4];
+// N searches, each for the first row j in [0, nsc) of the scoop at union row
+// e with fl32(K - deltat*k) > thr[i] (>= when ge[i]), nsc if none, in
+// lockstep with a fixed depth: every step tests row lo + step - 1 of each (a
+// row at or past nsc counts as passing), so the N independent LDS reads of a
+// step are in flight together.  top = the smallest power of two >= nsc.
+template <int N>
+__device__ __forceinline__ void gf_firstN(const GfLds &L, int e, int nsc, int top, float mdk,
+                                          float kf, const float *thr, const bool *ge,
+                                          int *out) {
+  int lo[N];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < N; ++i) lo[i] = 0;
+  for (int step = top; step > 0; step >>= 1) {
+    float c[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
       const int j = min(lo[i] + step - 1, nsc - 1);
       c[i] = fmaf(mdk, kf, L.K[e + j]);
     }
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const bool fail = (lo[i] + step <= nsc) & !(c[i] > thr[i]);
+    for (int i = 0; i < N; ++i) {
+      const bool pass = ge[i] ? c[i] >= thr[i] : c[i] > thr[i];
+      const bool fail = (lo[i] + step <= nsc) & !pass;
       lo[i] += fail ? step : 0;
     }
   }
 #pragma unroll
-  for (int i = 0; i < 4; ++i) out[i] = lo[i];
+  for (int i = 0; i < N; ++i) out[i] = lo[i];
 }
 
 // From one L to the next a region boundary (the first row at or after which
@@ -624,9 +616,27 @@ __global__ __launch_bounds__(kGfThreads, 4) void gtls_fast_kernel(
       const float dl = mode ? (float)((double)W / 4.0 / (double)(float)nl)
                             : (float)((double)W * 0.45 / (double)(float)nl);
       const float tEnd = (float)(0.5 * (double)W), tStart = (float)(-0.5 * (double)W);
-      const int nSx = gf_first(L, e, nsc, mdk, kf, tStart, 0);
-      const int mlo = gf_first(L, e, nsc, mdk, kf, tStart, 1);
-      const int eE = gf_first(L, e, nsc, mdk, kf, tEnd, 0);
+      // the region thresholds of step mm (the L grid's mm-th L)
+      auto thresholds = [&](int mm, float &Lf, float *thr) {
+        Lf = (mode ? (float)(mm + 1) : (float)mm) * dl;
+        thr[0] = (float)((double)(-W) * 0.25 - (double)Lf * 0.5);  // tAB
+        thr[1] = (float)((double)(-W) * 0.25 + (double)Lf * 0.5);  // tBC
+        thr[2] = -thr[1];                                          // tCD
+        thr[3] = -thr[0];                                          // tDE
+      };
+      // the window's ends (nSx: > tStart, mlo: >= tStart, eE: > tEnd) and
+      // the first L's four region ends, seven searches in lockstep
+      int bnd0[4], nSx, mlo, eE;
+      {
+        float th[7], Lf0;
+        th[0] = tStart; th[1] = tStart; th[2] = tEnd;
+        thresholds(0, Lf0, th + 3);
+        const bool ge[7] = {false, true, false, false, false, false, false};
+        int r[7];
+        gf_firstN<7>(L, e, nsc, top, mdk, kf, th, ge, r);
+        nSx = r[0]; mlo = r[1]; eE = r[2];
+        bnd0[0] = r[3]; bnd0[1] = r[4]; bnd0[2] = r[5]; bnd0[3] = r[6];
+      }
       // mean over the closed window (valid rows with tStart <= ct <= tEnd)
       const GfSum M = gf_sum(L, e + mlo, e + eE, sft);
       const double meanp = M.n > 0.0 ? M.x / M.n : -xref;  // x' units (mean 0 if empty)
@@ -639,19 +649,16 @@ __global__ __launch_bounds__(kGfThreads, 4) void gtls_fast_kernel(
         const double Y2 = Wn.x2 - 2.0 * meanp * Wn.x + n * meanp * meanp;
         // the L sweep, with the moments' shift to epoch coordinates elided
         // when the epoch is the tile's reference (one epoch per tile)
+        const double px0 = L.P[5 * (size_t)e];  // the scoop's first prefix row
+        const int v0 = L.V[e];
         auto sweep = [&](auto shift) {
         constexpr bool kSh = decltype(shift)::value;
-        int bnd[4] = {0, 0, 0, 0};
+        int bnd[4] = {bnd0[0], bnd0[1], bnd0[2], bnd0[3]};
         for (int mm = 0; mm < nl; ++mm) {
-          const float Lf = (mode ? (float)(mm + 1) : (float)mm) * dl;
+          float Lf, thr[4];
+          thresholds(mm, Lf, thr);
           const double dL = (double)Lf;
-          const float tAB = (float)((double)(-W) * 0.25 - (double)Lf * 0.5);
-          const float tBC = (float)((double)(-W) * 0.25 + (double)Lf * 0.5);
-          const float tCD = -tBC, tDE = -tAB;
-          const float thr[4] = {tAB, tBC, tCD, tDE};
-          if (mm == 0) {
-            gf_first4(L, e, nsc, top, mdk, kf, thr, bnd);
-          } else {
+          if (mm > 0) {
             // L grows with mm: tAB and tCD fall (their rows move down), tBC
             // and tDE rise (up)
             bnd[0] = gf_down(L, e, nsc, mdk, kf, thr[0], bnd[0]);
@@ -669,12 +676,14 @@ __global__ __launch_bounds__(kGfThreads, 4) void gtls_fast_kernel(
           // [0, eA - nSx) for the time tests of rows [nSx, eA)
           double sAq;
           const int cnt = eA - nSx;
+          // the quirk rows' prefix row end, read with the region rows (before
+          // the branch: one LDS round trip fewer per trial)
+          const double qx = L.P[5 * (size_t)(e + cnt)];
+          const int qv = L.V[e + cnt];
           if ((int)A.n == cnt) {
             // every tested row is valid: a prefix difference over the quirk
             // rows (masked rows there carry flux 0, i.e. x' = -x_ref)
-            const double *pa = L.P + 5 * (size_t)e, *pb = L.P + 5 * (size_t)(e + cnt);
-            const int vq = L.V[e + cnt] - L.V[e];
-            sAq = (pb[0] - pa[0]) - A.n * meanp + xref * (double)(vq - cnt);
+            sAq = (qx - px0) - A.n * meanp + xref * (double)(qv - v0 - cnt);
           } else {
             double sr = 0.0;
             for (int i = 0; i < cnt; ++i)
