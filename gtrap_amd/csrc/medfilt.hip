@@ -345,6 +345,23 @@ __global__ __launch_bounds__(256) void medfilt_block_kernel(
   medfilt_block_run<RM, B>(x + q, y + q, (size_t)nq, n, i0, i1, mode);
 }
 
+// ragged batch (serving), r = RM: lane g filters segment g of the segment list
+template <int RM, int B>
+__global__ __launch_bounds__(256) void medfilt_ragged_block_kernel(
+    const float *__restrict__ x, float *__restrict__ y, const int64_t *__restrict__ off,
+    const int32_t *__restrict__ len, const int32_t *__restrict__ seg_curve,
+    const int32_t *__restrict__ seg_start, const int32_t *__restrict__ nseg_dev, int mode,
+    int seg) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= *nseg_dev) return;
+  const int c = seg_curve[g];
+  const int n = len[c];
+  const int i0 = seg_start[g];
+  const int i1 = min(i0 + seg, n);
+  if (i0 >= i1) return;
+  medfilt_block_run<RM, B>(x + off[c], y + off[c], 1, n, i0, i1, mode);
+}
+
 // ---- wide windows in registers: one sorted window over a group of G lanes ----
 //
 // Lane gp (0..G-1) of a group holds slots [gp*WL, (gp+1)*WL) of one sorted
@@ -795,6 +812,16 @@ int medfilt_ragged_launch(const float *x, float *y, const int64_t *off, const in
     hipLaunchKernelGGL(medfilt_ragged_wide_kernel, dim3((unsigned)max_segs), dim3(kWideTile), shm,
                        st, x, y, off, len, sc, ss, nseg, (int)r, mode, seg);
     return check_launch("medfilt_ragged_wide_kernel");
+  }
+  if ((int)r == 64 || (int)r == 24) {  // blocked selection, as in medfilt_launch
+    const dim3 grid((unsigned)((max_segs + 255) / 256));
+    if ((int)r == 64)
+      hipLaunchKernelGGL((medfilt_ragged_block_kernel<64, 24>), grid, dim3(256), 0, st, x, y, off,
+                         len, sc, ss, nseg, mode, seg);
+    else
+      hipLaunchKernelGGL((medfilt_ragged_block_kernel<24, 16>), grid, dim3(256), 0, st, x, y, off,
+                         len, sc, ss, nseg, mode, seg);
+    return check_launch("medfilt_ragged_block_kernel");
   }
   int rmax = 64;
   for (int b : kBuckets)
