@@ -165,3 +165,22 @@ def test_tlsfit_repeated_refits_reuse_buffers(gpu):
         assert (W, L, T0, H) == (Wl[w], Ll[l], tl[k], Hd[w, l, k])
         with np.errstate(all="ignore"):
             np.testing.assert_array_equal(xpre, tls.gen_trapzoid(t, H, W, L, T0) + off)
+
+
+def test_tlsfit_unsorted_time(gpu):
+    """Shuffled samples: the grid scans every sample and the model curve takes
+    the reference's mask form; the choice and xpre equal the sorted run's."""
+    from gtrap_amd import tls
+    rng = np.random.default_rng(9)
+    t = np.sort(rng.uniform(0, 20, 3000))
+    x = 1.0 + 1e-3 * rng.standard_normal(3000) + tls.gen_trapzoid(t, -4e-3, 1.0, 0.2, 9.1)
+    Wl = np.linspace(0.8, 1.2, 5)
+    Ll = np.linspace(0.0, 0.4, 5)
+    tl = np.linspace(8.5, 9.7, 25)
+    a = tls.tlsfit(t, x, Wl, Ll, tl)
+    perm = rng.permutation(3000)
+    b = tls.tlsfit(t[perm], x[perm], Wl, Ll, tl)
+    assert a[2:5] == b[2:5] and abs(a[4] - 9.1) < 0.1
+    np.testing.assert_allclose([a[1], a[5]], [b[1], b[5]], rtol=1e-9)
+    with np.errstate(all="ignore"):
+        np.testing.assert_array_equal(b[0], tls.gen_trapzoid(t[perm], b[1], b[2], b[3], b[4]) + b[5])
