@@ -324,7 +324,7 @@ constexpr int kTfMaxTile = 144;  // epochs per staged tile: with the reference a
 constexpr int kTfMaxBuckets = 30000;  // phase buckets (LDS counters, <= 120 KB)
 constexpr int kTfSortChunk = 7680;    // samples per in-LDS bucket-sort chunk (90 KB + ends)
 constexpr int kTfMaxBreaks = 128;     // sort chunks per period (longer curves: one bitonic sort)
-constexpr int kTfUnroll = 8;          // global loads in flight per thread in the sort loops
+constexpr int kTfUnroll = 16;         // global loads in flight per thread in the sort loops (8: 0.25% slower)
 
 // Static LDS of a workgroup.  The per-(W, boundary) accumulators of the chunked
 // sweep and the per-(epoch, W) window table are sized at launch (dynamic LDS).
